@@ -1,0 +1,661 @@
+// api.hip -- C ABI (include/glsu.h) over the gfx950 kernels.
+//
+// Host-side restatement of the reference's class logic:
+//   WFT2_CUDAF ctor/operator()   WFT/WFT2_CUDAf.cu:849-1173  (WFF only)
+//   DPRA_CUDAF ctor/dpra_per_frame/update_ref_phi  DPRA/dpra_cudaf.cu:96-290
+//   DPRA_CPU::operator()         DPRA/dpra_cpu.cpp:58-89 (frame loop, i % rr)
+// Differences by design: status codes instead of printing and returning
+// (size mismatch, DPRA/dpra_cpu.cpp:135-139) or throwing int -1 (padding,
+// WFT/WFT2_CPU.cpp:69-73); no host synchronisation per frame unless the
+// caller asks for a time; WFF parameters exposed instead of hard-coded.
+#include "../../include/glsu.h"
+#include "glsu_internal.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+using namespace glsu;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(GLSU_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));     \
+  } while (0)
+
+static int check_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+    return fail(GLSU_ERR_DEVICE, "no HIP device visible");
+  return GLSU_OK;
+}
+
+static int elem_size(int dtype) {
+  switch (dtype) {
+    case GLSU_U8: return 1;
+    case GLSU_F32: return 4;
+    case GLSU_F64: return 8;
+    default: return 0;
+  }
+}
+
+// NULL selects the default (null) stream, which the reference's GPU classes use
+// (DPRA/dpra_cudaf.cu:207-284 launches on stream 0); any other value is used
+// as given and never destroyed by the library.
+static int make_stream(void* user, hipStream_t* s, bool* own) {
+  *s = (hipStream_t)user;
+  *own = false;
+  return GLSU_OK;
+}
+
+extern "C" const char* glsu_last_error(void) { return g_err.c_str(); }
+extern "C" const char* glsu_version(void) { return "glsu 0.1 (gfx950)"; }
+
+extern "C" int glsu_device_count(int* count) {
+  if (!count) return fail(GLSU_ERR_ARG, "null count");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return GLSU_OK;
+}
+
+extern "C" int glsu_wff_params_default(glsu_wff_params* p) {
+  if (!p) return fail(GLSU_ERR_ARG, "null params");
+  // WFT/WFT2_CPU.cpp:43-54
+  p->sigma_x = 10.0;
+  p->wxl = -2 - 3.0 / p->sigma_x;
+  p->wxi = 1.0 / p->sigma_x;
+  p->wxh = 2 + 3.0 / p->sigma_x;
+  p->sigma_y = 10.0;
+  p->wyl = -2 - 3.0 / p->sigma_y;
+  p->wyi = 1.0 / p->sigma_y;
+  p->wyh = 2 + 3.0 / p->sigma_y;
+  p->thr = -1;
+  return GLSU_OK;
+}
+
+extern "C" int glsu_wff_params_dpra_reference(glsu_wff_params* p) {
+  if (!p) return fail(GLSU_ERR_ARG, "null params");
+  // DPRA/dpra_cpu.cpp:23-25: float literals widened to double
+  p->sigma_x = 20; p->wxl = (double)-0.15f; p->wxh = (double)0.15f; p->wxi = (double)0.05f;
+  p->sigma_y = 20; p->wyl = (double)-0.15f; p->wyh = (double)0.15f; p->wyi = (double)0.05f;
+  p->thr = 15;
+  return GLSU_OK;
+}
+
+extern "C" int glsu_fft_plans(int* ids, int* lanes, int* elems, int* lengths, int cap, int* count) {
+  const auto& v = fft_plans();
+  int n = 0;
+  for (auto& p : v) {
+    if (n < cap) {
+      if (ids) ids[n] = p.id;
+      if (lanes) lanes[n] = p.T;
+      if (elems) elems[n] = p.E;
+      if (lengths) lengths[n] = p.N;
+    }
+    ++n;
+  }
+  if (count) *count = n;
+  return GLSU_OK;
+}
+
+// Padded size rule of the reference (WFT/WFT2_CPU.cpp:710-724 + WFT/WFT.h:18-32):
+// the constructor fails when OPT(H + 2s) or OPT(W + 2s) exceeds 4096.  The
+// library pads differently (any N >= size + s is exact) but keeps the same
+// acceptance rule so that callers see the same failures.
+static bool reference_padding_ok(int W, int H, int sx, int sy) {
+  return W + 2 * sx <= 4096 && H + 2 * sy <= 4096;
+}
+
+static int default_plan(int need) {
+  auto c = fft_candidates(need);
+  if (c.empty()) return -1;
+  // prefer one transform per wavefront unless a narrower plan is much shorter
+  int best64 = -1, bestN64 = 0, first = c[0], firstN = 0;
+  for (int id : c)
+    for (auto& p : fft_plans())
+      if (p.id == id) {
+        if (id == first) firstN = p.N;
+        if (p.T == 64 && best64 < 0) { best64 = id; bestN64 = p.N; }
+      }
+  if (best64 < 0) return first;
+  return (firstN < 0.8 * bestN64) ? first : best64;
+}
+
+static int plan_common(WffPlan& p, int W, int H, const glsu_wff_params* pr, int planx, int plany) {
+  if (W < 1 || H < 1) return fail(GLSU_ERR_ARG, "invalid size %dx%d", W, H);
+  if (!(pr->wxi > 0) || !(pr->wyi > 0) || !(pr->sigma_x > 0) || !(pr->sigma_y > 0))
+    return fail(GLSU_ERR_ARG, "invalid WFF parameters");
+  const int sx = (int)std::round(3 * pr->sigma_x), sy = (int)std::round(3 * pr->sigma_y);
+  if (!reference_padding_ok(W, H, sx, sy))
+    return fail(GLSU_ERR_PADDING,
+                "FFT padding is out of range [4096]. Shrink the size of either the image or the "
+                "Gaussian Window!");
+  if (planx < 0) planx = default_plan(W + sx);
+  if (plany < 0) plany = default_plan(H + sy);
+  if (planx < 0 || plany < 0)
+    return fail(GLSU_ERR_PADDING, "no compiled FFT length covers %dx%d + half window", W, H);
+  std::string err;
+  int rc = wff_plan_init(p, W, H, pr->sigma_x, pr->wxl, pr->wxh, pr->wxi, pr->sigma_y, pr->wyl,
+                         pr->wyh, pr->wyi, pr->thr, planx, plany, err);
+  if (rc == -2) return fail(GLSU_ERR_ARG, "%s", err.c_str());
+  if (rc) return fail(GLSU_ERR_DEVICE, "%s", err.c_str());
+  return GLSU_OK;
+}
+
+// =============================================================== WFT2
+struct glsu_wft2 {
+  WffPlan p;
+  hipStream_t s = nullptr;
+  bool own = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float2* d_in = nullptr;   // staging for the host-pointer form
+  float2* d_out = nullptr;
+};
+
+extern "C" int glsu_wft2_create_plan(glsu_wft2** out, int W, int H, int type,
+                                     const glsu_wff_params* pr, void* stream, int planx,
+                                     int plany) {
+  if (!out || !pr) return fail(GLSU_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (type == GLSU_WFR) return fail(GLSU_ERR_UNSUPPORTED, "WFR2 mode is not provided");
+  if (type != GLSU_WFF) return fail(GLSU_ERR_ARG, "unknown WFT type %d", type);
+  if (int rc = check_device()) return rc;
+  auto* h = new glsu_wft2();
+  int rc = plan_common(h->p, W, H, pr, planx, plany);
+  if (rc) { delete h; return rc; }
+  rc = make_stream(stream, &h->s, &h->own);
+  if (rc) { wff_plan_free(h->p); delete h; return rc; }
+  hipEventCreate(&h->e0);
+  hipEventCreate(&h->e1);
+  *out = h;
+  return GLSU_OK;
+}
+
+extern "C" int glsu_wft2_create(glsu_wft2** out, int W, int H, int type, const glsu_wff_params* p,
+                                void* stream) {
+  return glsu_wft2_create_plan(out, W, H, type, p, stream, -1, -1);
+}
+
+static int wft2_enqueue(glsu_wft2* h, const float2* f, float2* out) {
+  if (!h->p.thr_ready) {
+    HIP_TRY(launch_thr(h->p, f, h->s));
+    h->p.thr_ready = true;
+  }
+  HIP_TRY(launch_rows_fwd(h->p, f, h->s));
+  HIP_TRY(launch_cols(h->p, h->s));
+  HIP_TRY(launch_rows_inv_filtered(h->p, out, h->s));
+  return GLSU_OK;
+}
+
+extern "C" int glsu_wft2_compute(glsu_wft2* h, const void* d_f, void* d_out, double* ms) {
+  if (!h || !d_f || !d_out) return fail(GLSU_ERR_ARG, "null argument");
+  if (ms) HIP_TRY(hipEventRecord(h->e0, h->s));
+  if (int rc = wft2_enqueue(h, (const float2*)d_f, (float2*)d_out)) return rc;
+  if (ms) {
+    HIP_TRY(hipEventRecord(h->e1, h->s));
+    HIP_TRY(hipEventSynchronize(h->e1));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, h->e0, h->e1));
+    *ms = t;
+  }
+  return GLSU_OK;
+}
+
+extern "C" int glsu_wft2_compute_host(glsu_wft2* h, const float* hf, float* hout, double* ms) {
+  if (!h || !hf || !hout) return fail(GLSU_ERR_ARG, "null argument");
+  const size_t bytes = sizeof(float2) * (size_t)h->p.W * h->p.H;
+  if (!h->d_in) {
+    HIP_TRY(hipMalloc((void**)&h->d_in, bytes));
+    HIP_TRY(hipMalloc((void**)&h->d_out, bytes));
+  }
+  if (ms) HIP_TRY(hipEventRecord(h->e0, h->s));
+  HIP_TRY(hipMemcpyAsync(h->d_in, hf, bytes, hipMemcpyHostToDevice, h->s));
+  if (int rc = wft2_enqueue(h, h->d_in, h->d_out)) return rc;
+  HIP_TRY(hipMemcpyAsync(hout, h->d_out, bytes, hipMemcpyDeviceToHost, h->s));
+  if (ms) HIP_TRY(hipEventRecord(h->e1, h->s));
+  HIP_TRY(hipStreamSynchronize(h->s));
+  if (ms) {
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, h->e0, h->e1));
+    *ms = t;
+  }
+  return GLSU_OK;
+}
+
+static double read_thr(const WffPlan& p, hipStream_t s) {
+  if (!p.thr_ready) return std::numeric_limits<double>::quiet_NaN();
+  float t = 0;
+  hipMemcpyAsync(&t, p.d_thr, sizeof(float), hipMemcpyDeviceToHost, s);
+  hipStreamSynchronize(s);
+  return t;
+}
+
+extern "C" int glsu_wft2_info(glsu_wft2* h, int* nx, int* ny, int* sx, int* sy, int* nwx, int* nwy,
+                              double* thr) {
+  if (!h) return fail(GLSU_ERR_ARG, "null handle");
+  if (nx) *nx = h->p.Nx;
+  if (ny) *ny = h->p.Ny;
+  if (sx) *sx = h->p.sx;
+  if (sy) *sy = h->p.sy;
+  if (nwx) *nwx = h->p.nwx;
+  if (nwy) *nwy = h->p.nwy;
+  if (thr) *thr = read_thr(h->p, h->s);
+  return GLSU_OK;
+}
+
+extern "C" int glsu_wft2_destroy(glsu_wft2* h) {
+  if (!h) return GLSU_OK;
+  hipStreamSynchronize(h->s);
+  wff_plan_free(h->p);
+  hipFree(h->d_in);
+  hipFree(h->d_out);
+  hipEventDestroy(h->e0);
+  hipEventDestroy(h->e1);
+  if (h->own) hipStreamDestroy(h->s);
+  delete h;
+  return GLSU_OK;
+}
+
+// =============================================================== DPRA
+struct GraphKey {
+  const void* frames;
+  int nframes, dtype;
+  float* dphi;
+  const float* phiref;
+  bool thr_ready;
+  bool fresh;
+  bool operator<(const GraphKey& o) const {
+    return std::tie(frames, nframes, dtype, dphi, phiref, thr_ready, fresh) <
+           std::tie(o.frames, o.nframes, o.dtype, o.dphi, o.phiref, o.thr_ready, o.fresh);
+  }
+};
+
+struct GraphEntry {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  // host-side pointer state at the end of the captured sequence
+  float *phiref, *phicurr, *dref_hi, *dref_lo, *dphi_hi, *dphi_lo;
+  bool fresh;
+};
+
+struct glsu_dpra {
+  int W = 0, H = 0, rr = 1;
+  WffPlan p;
+  hipStream_t s = nullptr;
+  bool own = false;
+  float *phiref = nullptr, *phicurr = nullptr;
+  float *dref_hi = nullptr, *dref_lo = nullptr, *dphi_hi = nullptr, *dphi_lo = nullptr;
+  float2* z = nullptr;
+  bool fresh = true;  // phicurr/dphi hold values not yet taken into the reference
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  void* d_stage = nullptr;
+  size_t stage_bytes = 0;
+  float* d_dphi_stage = nullptr;
+  hipStream_t cap = nullptr;  // capture stream for glsu_dpra_run_graph
+  std::map<GraphKey, GraphEntry> graphs;
+};
+
+static int dpra_alloc(glsu_dpra* h) {
+  const size_t n = (size_t)h->W * h->H;
+  HIP_TRY(hipMalloc((void**)&h->phiref, n * sizeof(float)));
+  HIP_TRY(hipMalloc((void**)&h->phicurr, n * sizeof(float)));
+  HIP_TRY(hipMalloc((void**)&h->dref_hi, n * sizeof(float)));
+  HIP_TRY(hipMalloc((void**)&h->dref_lo, n * sizeof(float)));
+  HIP_TRY(hipMalloc((void**)&h->dphi_hi, n * sizeof(float)));
+  HIP_TRY(hipMalloc((void**)&h->dphi_lo, n * sizeof(float)));
+  HIP_TRY(hipMalloc((void**)&h->z, n * sizeof(float2)));
+  // DPRA_CPU ctor: phicurr = 0, dphiref = 0 (DPRA/dpra_cpu.cpp:17-20)
+  HIP_TRY(hipMemset(h->phicurr, 0, n * sizeof(float)));
+  HIP_TRY(hipMemset(h->dref_hi, 0, n * sizeof(float)));
+  HIP_TRY(hipMemset(h->dref_lo, 0, n * sizeof(float)));
+  HIP_TRY(hipMemset(h->dphi_hi, 0, n * sizeof(float)));
+  HIP_TRY(hipMemset(h->dphi_lo, 0, n * sizeof(float)));
+  return GLSU_OK;
+}
+
+static void dpra_free(glsu_dpra* h) {
+  for (auto& kv : h->graphs) {
+    hipGraphExecDestroy(kv.second.x);
+    hipGraphDestroy(kv.second.g);
+  }
+  h->graphs.clear();
+  hipFree(h->phiref); hipFree(h->phicurr);
+  hipFree(h->dref_hi); hipFree(h->dref_lo); hipFree(h->dphi_hi); hipFree(h->dphi_lo);
+  hipFree(h->z); hipFree(h->d_stage); hipFree(h->d_dphi_stage);
+  wff_plan_free(h->p);
+  if (h->e0) hipEventDestroy(h->e0);
+  if (h->e1) hipEventDestroy(h->e1);
+  if (h->own && h->s) hipStreamDestroy(h->s);
+  if (h->cap) hipStreamDestroy(h->cap);
+}
+
+// phase values are only used through cos/sin and differences; keep them in
+// (-pi, pi] so FP32 storage keeps full relative precision
+static float wrap_d(double v) { return (float)std::atan2(std::sin(v), std::cos(v)); }
+
+template <typename PT>
+static int dpra_create_impl(glsu_dpra** out, const PT* phi0, int W, int H, int rr,
+                            const glsu_wff_params* pr, void* stream, int planx, int plany) {
+  if (!out || !phi0) return fail(GLSU_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (rr < 1) return fail(GLSU_ERR_ARG, "reference update rate must be >= 1");
+  if (int rc = check_device()) return rc;
+  glsu_wff_params def;
+  if (!pr) {
+    glsu_wff_params_dpra_reference(&def);
+    pr = &def;
+  }
+  auto* h = new glsu_dpra();
+  h->W = W; h->H = H; h->rr = rr;
+  int rc = plan_common(h->p, W, H, pr, planx, plany);
+  if (rc) { delete h; return rc; }
+  rc = make_stream(stream, &h->s, &h->own);
+  if (!rc) rc = dpra_alloc(h);
+  if (rc) { dpra_free(h); delete h; return rc; }
+  std::vector<float> ph((size_t)W * H);
+  for (size_t i = 0; i < ph.size(); ++i) ph[i] = wrap_d((double)phi0[i]);
+  if (hipMemcpy(h->phiref, ph.data(), ph.size() * sizeof(float), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    dpra_free(h);
+    delete h;
+    return fail(GLSU_ERR_DEVICE, "upload of phi0 failed");
+  }
+  hipEventCreate(&h->e0);
+  hipEventCreate(&h->e1);
+  *out = h;
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_create(glsu_dpra** out, const double* phi0, int W, int H, int rr,
+                                const glsu_wff_params* p, void* stream) {
+  return dpra_create_impl(out, phi0, W, H, rr, p, stream, -1, -1);
+}
+extern "C" int glsu_dpra_create_f32(glsu_dpra** out, const float* phi0, int W, int H, int rr,
+                                    const glsu_wff_params* p, void* stream) {
+  return dpra_create_impl(out, phi0, W, H, rr, p, stream, -1, -1);
+}
+extern "C" int glsu_dpra_create_plan(glsu_dpra** out, const double* phi0, int W, int H, int rr,
+                                     const glsu_wff_params* p, void* stream, int planx,
+                                     int plany) {
+  return dpra_create_impl(out, phi0, W, H, rr, p, stream, planx, plany);
+}
+
+// one frame: LS -> (threshold on first call) -> rows fwd -> columns -> rows inv + update
+static int dpra_enqueue(glsu_dpra* h, const void* frame, int dtype, float* d_dphi,
+                        hipStream_t st) {
+  WffPlan& p = h->p;
+  HIP_TRY(launch_dpra_ls(frame, dtype, h->phiref, h->z, h->W, h->H, h->W, st));
+  if (!p.thr_ready) {
+    HIP_TRY(launch_thr(p, h->z, st));
+    p.thr_ready = true;
+  }
+  HIP_TRY(launch_rows_fwd(p, h->z, st));
+  HIP_TRY(launch_cols(p, st));
+  UpdateArgs u{h->phiref, h->dref_hi, h->dref_lo, h->phicurr, h->dphi_hi, h->dphi_lo, d_dphi,
+               nullptr};
+  HIP_TRY(launch_rows_inv_update(p, u, st));
+  h->fresh = true;
+  return GLSU_OK;
+}
+
+// update_ref_phi, DPRA/dpra_cpu.cpp:254-258: phiref <- phicurr, dphiref <- dphi.
+// Realised as a buffer swap; a repeated call without a new frame is a no-op,
+// which is what repeated copies of unchanged buffers amount to.
+static void dpra_swap(glsu_dpra* h) {
+  if (!h->fresh) return;
+  std::swap(h->phiref, h->phicurr);
+  std::swap(h->dref_hi, h->dphi_hi);
+  std::swap(h->dref_lo, h->dphi_lo);
+  h->fresh = false;
+}
+
+extern "C" int glsu_dpra_frame(glsu_dpra* h, const void* d_frame, int dtype, float* d_dphi,
+                               float* d_phicurr, double* ms) {
+  if (!h || !d_frame) return fail(GLSU_ERR_ARG, "null argument");
+  if (!elem_size(dtype)) return fail(GLSU_ERR_ARG, "unknown frame dtype %d", dtype);
+  if (ms) HIP_TRY(hipEventRecord(h->e0, h->s));
+  if (int rc = dpra_enqueue(h, d_frame, dtype, d_dphi, h->s)) return rc;
+  if (d_phicurr)
+    HIP_TRY(hipMemcpyAsync(d_phicurr, h->phicurr, sizeof(float) * (size_t)h->W * h->H,
+                           hipMemcpyDeviceToDevice, h->s));
+  if (ms) {
+    HIP_TRY(hipEventRecord(h->e1, h->s));
+    HIP_TRY(hipEventSynchronize(h->e1));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, h->e0, h->e1));
+    *ms = t;
+  }
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_frame_host(glsu_dpra* h, const void* hf, int dtype, float* hdphi,
+                                    double* ms) {
+  if (!h || !hf) return fail(GLSU_ERR_ARG, "null argument");
+  const int es = elem_size(dtype);
+  if (!es) return fail(GLSU_ERR_ARG, "unknown frame dtype %d", dtype);
+  const size_t n = (size_t)h->W * h->H;
+  if (h->stage_bytes < n * es) {
+    hipFree(h->d_stage);
+    HIP_TRY(hipMalloc(&h->d_stage, n * es));
+    h->stage_bytes = n * es;
+  }
+  if (!h->d_dphi_stage) HIP_TRY(hipMalloc((void**)&h->d_dphi_stage, n * sizeof(float)));
+  if (ms) HIP_TRY(hipEventRecord(h->e0, h->s));
+  HIP_TRY(hipMemcpyAsync(h->d_stage, hf, n * es, hipMemcpyHostToDevice, h->s));
+  if (int rc = dpra_enqueue(h, h->d_stage, dtype, h->d_dphi_stage, h->s)) return rc;
+  if (hdphi)
+    HIP_TRY(hipMemcpyAsync(hdphi, h->d_dphi_stage, n * sizeof(float), hipMemcpyDeviceToHost, h->s));
+  if (ms) HIP_TRY(hipEventRecord(h->e1, h->s));
+  HIP_TRY(hipStreamSynchronize(h->s));
+  if (ms) {
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, h->e0, h->e1));
+    *ms = t;
+  }
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_update_ref(glsu_dpra* h) {
+  if (!h) return fail(GLSU_ERR_ARG, "null handle");
+  dpra_swap(h);
+  return GLSU_OK;
+}
+
+static int dpra_run_enqueue(glsu_dpra* h, const void* frames, int nframes, int dtype,
+                            float* dphi_all, hipStream_t st) {
+  const size_t n = (size_t)h->W * h->H;
+  const int es = elem_size(dtype);
+  for (int i = 0; i < nframes; ++i) {
+    const char* f = (const char*)frames + (size_t)i * n * es;
+    if (int rc = dpra_enqueue(h, f, dtype, dphi_all ? dphi_all + (size_t)i * n : nullptr, st))
+      return rc;
+    if (i % h->rr == 0) dpra_swap(h);
+  }
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_run(glsu_dpra* h, const void* frames, int nframes, int dtype,
+                             float* dphi_all, double* ms) {
+  if (!h || !frames || nframes < 0) return fail(GLSU_ERR_ARG, "invalid argument");
+  if (!elem_size(dtype)) return fail(GLSU_ERR_ARG, "unknown frame dtype %d", dtype);
+  if (ms) HIP_TRY(hipEventRecord(h->e0, h->s));
+  if (int rc = dpra_run_enqueue(h, frames, nframes, dtype, dphi_all, h->s)) return rc;
+  if (ms) {
+    HIP_TRY(hipEventRecord(h->e1, h->s));
+    HIP_TRY(hipEventSynchronize(h->e1));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, h->e0, h->e1));
+    *ms = t;
+  }
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_run_graph(glsu_dpra* h, const void* frames, int nframes, int dtype,
+                                   float* dphi_all, double* ms) {
+  if (!h || !frames || nframes < 0) return fail(GLSU_ERR_ARG, "invalid argument");
+  if (!elem_size(dtype)) return fail(GLSU_ERR_ARG, "unknown frame dtype %d", dtype);
+  GraphKey key{frames, nframes, dtype, dphi_all, h->phiref, h->p.thr_ready, h->fresh};
+  auto it = h->graphs.find(key);
+  if (it == h->graphs.end()) {
+    GraphEntry ge;
+    // capture on a private stream (the null stream cannot be captured); the
+    // graph is launched on the handle's stream below
+    if (!h->cap) HIP_TRY(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
+    HIP_TRY(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
+    int rc = dpra_run_enqueue(h, frames, nframes, dtype, dphi_all, h->cap);
+    hipGraph_t g = nullptr;
+    hipError_t ee = hipStreamEndCapture(h->cap, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (ee != hipSuccess) return fail(GLSU_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(ee));
+    ge.g = g;
+    HIP_TRY(hipGraphInstantiate(&ge.x, g, nullptr, nullptr, 0));
+    ge.phiref = h->phiref; ge.phicurr = h->phicurr;
+    ge.dref_hi = h->dref_hi; ge.dref_lo = h->dref_lo;
+    ge.dphi_hi = h->dphi_hi; ge.dphi_lo = h->dphi_lo;
+    ge.fresh = h->fresh;
+    it = h->graphs.emplace(key, ge).first;
+  } else {
+    // replay ends in the same host pointer state the capture ended in
+    h->phiref = it->second.phiref; h->phicurr = it->second.phicurr;
+    h->dref_hi = it->second.dref_hi; h->dref_lo = it->second.dref_lo;
+    h->dphi_hi = it->second.dphi_hi; h->dphi_lo = it->second.dphi_lo;
+    h->fresh = it->second.fresh;
+    h->p.thr_ready = true;
+  }
+  if (ms) HIP_TRY(hipEventRecord(h->e0, h->s));
+  HIP_TRY(hipGraphLaunch(it->second.x, h->s));
+  if (ms) {
+    HIP_TRY(hipEventRecord(h->e1, h->s));
+    HIP_TRY(hipEventSynchronize(h->e1));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, h->e0, h->e1));
+    *ms = t;
+  }
+  return GLSU_OK;
+}
+
+// Per-kernel device times over a frame loop (events between the kernels of
+// each frame).  ms[0..3] receive the summed times of the least-squares, row
+// forward, column and row inverse + update kernels.  Diagnostics only.
+extern "C" int glsu_dpra_profile(glsu_dpra* h, const void* frames, int nframes, int dtype,
+                                 float* dphi_all, double* ms) {
+  if (!h || !frames || !ms || nframes < 0) return fail(GLSU_ERR_ARG, "invalid argument");
+  const int es = elem_size(dtype);
+  if (!es) return fail(GLSU_ERR_ARG, "unknown frame dtype %d", dtype);
+  const size_t n = (size_t)h->W * h->H;
+  hipEvent_t ev[5];
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  for (int k = 0; k < 4; ++k) ms[k] = 0;
+  WffPlan& p = h->p;
+  for (int i = 0; i < nframes; ++i) {
+    const char* f = (const char*)frames + (size_t)i * n * es;
+    HIP_TRY(hipEventRecord(ev[0], h->s));
+    HIP_TRY(launch_dpra_ls(f, dtype, h->phiref, h->z, h->W, h->H, h->W, h->s));
+    if (!p.thr_ready) {
+      HIP_TRY(launch_thr(p, h->z, h->s));
+      p.thr_ready = true;
+    }
+    HIP_TRY(hipEventRecord(ev[1], h->s));
+    HIP_TRY(launch_rows_fwd(p, h->z, h->s));
+    HIP_TRY(hipEventRecord(ev[2], h->s));
+    HIP_TRY(launch_cols(p, h->s));
+    HIP_TRY(hipEventRecord(ev[3], h->s));
+    UpdateArgs u{h->phiref, h->dref_hi, h->dref_lo, h->phicurr, h->dphi_hi, h->dphi_lo,
+                 dphi_all ? dphi_all + (size_t)i * n : nullptr, nullptr};
+    HIP_TRY(launch_rows_inv_update(p, u, h->s));
+    HIP_TRY(hipEventRecord(ev[4], h->s));
+    h->fresh = true;
+    if (i % h->rr == 0) dpra_swap(h);
+    HIP_TRY(hipEventSynchronize(ev[4]));
+    for (int k = 0; k < 4; ++k) {
+      float t = 0;
+      HIP_TRY(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+      ms[k] += t;
+    }
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_get_state(glsu_dpra* h, float* phiref, float* dphiref) {
+  if (!h) return fail(GLSU_ERR_ARG, "null handle");
+  const size_t n = (size_t)h->W * h->H;
+  HIP_TRY(hipStreamSynchronize(h->s));
+  if (phiref) HIP_TRY(hipMemcpy(phiref, h->phiref, n * sizeof(float), hipMemcpyDeviceToHost));
+  if (dphiref) {
+    std::vector<float> hi(n), lo(n);
+    HIP_TRY(hipMemcpy(hi.data(), h->dref_hi, n * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(lo.data(), h->dref_lo, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) dphiref[i] = hi[i] + lo[i];
+  }
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_set_state(glsu_dpra* h, const float* phiref, const float* dphiref) {
+  if (!h) return fail(GLSU_ERR_ARG, "null handle");
+  const size_t n = (size_t)h->W * h->H;
+  HIP_TRY(hipStreamSynchronize(h->s));
+  if (phiref) {
+    std::vector<float> w(n);
+    for (size_t i = 0; i < n; ++i) w[i] = wrap_d((double)phiref[i]);
+    HIP_TRY(hipMemcpy(h->phiref, w.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  }
+  if (dphiref) {
+    HIP_TRY(hipMemcpy(h->dref_hi, dphiref, n * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(h->dref_lo, 0, n * sizeof(float)));
+  }
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_copy_z(glsu_dpra* h, void* d_z) {
+  if (!h || !d_z) return fail(GLSU_ERR_ARG, "null argument");
+  HIP_TRY(hipMemcpyAsync(d_z, h->z, sizeof(float2) * (size_t)h->W * h->H,
+                         hipMemcpyDeviceToDevice, h->s));
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_thr(glsu_dpra* h, double* thr) {
+  if (!h || !thr) return fail(GLSU_ERR_ARG, "null argument");
+  *thr = read_thr(h->p, h->s);
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_destroy(glsu_dpra* h) {
+  if (!h) return GLSU_OK;
+  hipStreamSynchronize(h->s);
+  dpra_free(h);
+  delete h;
+  return GLSU_OK;
+}
+
+extern "C" int glsu_gen_frames(float* d_out, int W, int H, int t0, int T, uint64_t seed, double amp0,
+                               double amp1, double carrier, double noise, void* stream) {
+  if (!d_out || W < 1 || H < 1 || T < 0) return fail(GLSU_ERR_ARG, "invalid argument");
+  if (int rc = check_device()) return rc;
+  HIP_TRY(launch_gen_frames(d_out, W, H, t0, T, seed, amp0, amp1, carrier, noise,
+                            (hipStream_t)stream));
+  return GLSU_OK;
+}

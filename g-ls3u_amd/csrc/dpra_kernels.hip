@@ -1,0 +1,222 @@
+// dpra_kernels.hip -- DPRA 3x3 least squares on gfx950, and the synthetic
+// fringe generator.
+//
+// Reference (DPRA/dpra_cpu.cpp:153-232): for every pixel p, over the clipped
+// 3x3 neighbourhood N(p) with c = cos(phiref), s = sin(phiref),
+//   A = [[9, Sc, Ss], [Sc, Scc, Scs], [Ss, Scs, Sss]]   (A00 = 9 even at borders)
+//   b = [SI, SIc, SIs],  x = A^-1 b,  Delta = atan2(-x2, x1),  z = e^{j Delta}.
+//
+// Same solution, better conditioned arithmetic.  Rotate the (c, s) basis by
+// the centre phase phi_p and shift the cosine column by one:
+//   w_q = [1, e_q, s'_q],  e_q = cos(d_q) - 1 = -2 sin^2(d_q/2),  s'_q = sin(d_q),
+//   d_q = phi_q - phi_p.
+// A = R^T M A'' M^T R with A'' = sum_q w_q w_q^T + (9 - n) w0 w0^T, w0 = [1,-1,0],
+// and b'' = sum_q I_q w_q, so A'' y = b'' gives x'_1 = y1, x'_2 = y2 and
+//   Delta = atan2(-y2, y1) - phi_p,  i.e.  z = (y1 - j y2)/|y| * e^{-j phi_p}.
+// The entries of A'' are O(d^2..d^4) quantities formed without cancellation;
+// the 9 sums are accumulated in FP32 with a compensated two-sum, the DC
+// unknown is eliminated (2x2 Schur complement) and the 2x2 system is solved by
+// Cramer's rule.  det(A) = 9 det2.  Where |det A| < 1e-6 * A00*A11*A22 the 2x2
+// solve is redone in FP64 from the compensated sums; where
+// det A <= 1e-10 * A00*A11*A22 the pixel is singular and z = 1 (the oracle's
+// defined behaviour for the case where LAPACKE_dsysv reports failure).
+#include "glsu_internal.h"
+
+#include <cstdint>
+
+namespace glsu {
+
+struct Comp {  // compensated FP32 accumulator (Knuth TwoSum)
+  float s, c;
+  __device__ __forceinline__ void init() { s = 0.f; c = 0.f; }
+  __device__ __forceinline__ void add(float x) {
+    const float t = s + x;
+    const float bp = t - s;
+    c += (s - (t - bp)) + (x - bp);
+    s = t;
+  }
+  __device__ __forceinline__ float f() const { return s + c; }
+  __device__ __forceinline__ double d() const { return (double)s + (double)c; }
+};
+
+template <typename InT>
+__device__ __forceinline__ float load_in(const InT* p, size_t i) {
+  return (float)p[i];
+}
+
+constexpr int DTX = 32, DTY = 8;
+
+template <typename InT>
+__global__ __launch_bounds__(DTX* DTY) void dpra_ls_kernel(const InT* __restrict__ I,
+                                                           const float* __restrict__ phr,
+                                                           float2* __restrict__ z, int W, int H,
+                                                           int pitch) {
+  __shared__ float sI[DTY + 2][DTX + 2];
+  __shared__ float sP[DTY + 2][DTX + 2];
+  const int tx = threadIdx.x % DTX, ty = threadIdx.x / DTX;
+  const int x0 = blockIdx.x * DTX - 1, y0 = blockIdx.y * DTY - 1;
+  for (int i = threadIdx.x; i < (DTX + 2) * (DTY + 2); i += DTX * DTY) {
+    const int ly = i / (DTX + 2), lx = i % (DTX + 2);
+    const int gx = x0 + lx, gy = y0 + ly;
+    float vi = 0.f, vp = 0.f;
+    if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+      vi = load_in(I, (size_t)gy * pitch + gx);
+      vp = phr[(size_t)gy * W + gx];
+    }
+    sI[ly][lx] = vi;
+    sP[ly][lx] = vp;
+  }
+  __syncthreads();
+  const int x = blockIdx.x * DTX + tx, y = blockIdx.y * DTY + ty;
+  if (x >= W || y >= H) return;
+
+  const float pp = sP[ty + 1][tx + 1];
+  Comp Se, Ss, See, Ses, Sss, SI, SIe, SIs;
+  Se.init(); Ss.init(); See.init(); Ses.init(); Sss.init(); SI.init(); SIe.init(); SIs.init();
+  int n = 0;
+#pragma unroll
+  for (int dy = -1; dy <= 1; ++dy) {
+#pragma unroll
+    for (int dx = -1; dx <= 1; ++dx) {
+      const int gx = x + dx, gy = y + dy;
+      if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
+      ++n;
+      const float d = sP[ty + 1 + dy][tx + 1 + dx] - pp;
+      float sh, ch;
+      sincosf(0.5f * d, &sh, &ch);
+      const float sq = 2.f * sh * ch;
+      const float e = -2.f * sh * sh;
+      const float Iq = sI[ty + 1 + dy][tx + 1 + dx];
+      Se.add(e); Ss.add(sq);
+      See.add(e * e); Ses.add(e * sq); Sss.add(sq * sq);
+      SI.add(Iq); SIe.add(Iq * e); SIs.add(Iq * sq);
+    }
+  }
+  float sp, cp;
+  sincosf(pp, &sp, &cp);
+  const float q = (float)(9 - n);
+  // diagonal of the unrotated A: A11 = sum c^2, A22 = sum s^2
+  const float s1e2 = (float)n + 2.f * Se.f() + See.f();   // sum (1+e)^2
+  const float s1es = Ss.f() + Ses.f();                     // sum (1+e) s'
+  const float A11 = cp * cp * s1e2 - 2.f * cp * sp * s1es + sp * sp * Sss.f();
+  const float A22 = sp * sp * s1e2 + 2.f * cp * sp * s1es + cp * cp * Sss.f();
+  const float dg = 9.f * A11 * A22;
+
+  // FP32 Schur complement of A'' on the DC unknown
+  const float a01 = Se.f() - q, a02 = Ss.f(), a11 = See.f() + q, a12 = Ses.f(), a22 = Sss.f();
+  const float b0 = SI.f(), b1 = SIe.f(), b2 = SIs.f();
+  const float s11 = a11 - a01 * a01 * (1.f / 9.f);
+  const float s12 = a12 - a01 * a02 * (1.f / 9.f);
+  const float s22 = a22 - a02 * a02 * (1.f / 9.f);
+  const float r1 = b1 - a01 * b0 * (1.f / 9.f);
+  const float r2 = b2 - a02 * b0 * (1.f / 9.f);
+  float det2 = s11 * s22 - s12 * s12;
+  float y1 = r1 * s22 - s12 * r2;
+  float y2 = s11 * r2 - s12 * r1;
+  bool singular = false;
+  if (!(fabsf(9.f * det2) >= 1e-6f * fabsf(dg))) {
+    // near-singular: FP64 solve from the compensated sums
+    const double qd = (double)(9 - n);
+    const double A01 = Se.d() - qd, A02 = Ss.d(), A11d = See.d() + qd, A12 = Ses.d(),
+                 A22d = Sss.d();
+    const double B0 = SI.d(), B1 = SIe.d(), B2 = SIs.d();
+    const double S11 = A11d - A01 * A01 / 9.0, S12 = A12 - A01 * A02 / 9.0,
+                 S22 = A22d - A02 * A02 / 9.0;
+    const double R1 = B1 - A01 * B0 / 9.0, R2 = B2 - A02 * B0 / 9.0;
+    const double D2 = S11 * S22 - S12 * S12;
+    const double Y1 = R1 * S22 - S12 * R2, Y2 = S11 * R2 - S12 * R1;
+    if (!(fabs(9.0 * D2) > 1e-10 * fabs((double)dg))) {
+      singular = true;
+    } else {
+      const double sg = D2 < 0 ? -1.0 : 1.0;
+      const double nr = sqrt(Y1 * Y1 + Y2 * Y2);
+      y1 = (float)(sg * Y1 / nr);
+      y2 = (float)(sg * Y2 / nr);
+      det2 = 1.f;
+    }
+  }
+  float2 out;
+  if (singular || (y1 == 0.f && y2 == 0.f)) {
+    out = make_float2(1.f, 0.f);
+  } else {
+    if (det2 < 0.f) { y1 = -y1; y2 = -y2; }
+    const float inv = rsqrtf(y1 * y1 + y2 * y2);
+    const float ur = y1 * inv, ui = -y2 * inv;      // e^{j theta'}
+    out = make_float2(ur * cp + ui * sp, ui * cp - ur * sp);  // * e^{-j phi_p}
+  }
+  z[(size_t)y * W + x] = out;
+}
+
+hipError_t launch_dpra_ls(const void* frame, int dtype, const float* phiref, float2* z, int W,
+                          int H, int pitch, hipStream_t s) {
+  dim3 grid((W + DTX - 1) / DTX, (H + DTY - 1) / DTY), block(DTX * DTY);
+  switch (dtype) {
+    case IN_U8:
+      hipLaunchKernelGGL(dpra_ls_kernel<uint8_t>, grid, block, 0, s, (const uint8_t*)frame,
+                         phiref, z, W, H, pitch);
+      break;
+    case IN_F32:
+      hipLaunchKernelGGL(dpra_ls_kernel<float>, grid, block, 0, s, (const float*)frame, phiref,
+                         z, W, H, pitch);
+      break;
+    case IN_F64:
+      hipLaunchKernelGGL(dpra_ls_kernel<double>, grid, block, 0, s, (const double*)frame,
+                         phiref, z, W, H, pitch);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic fringes: I = 128 + 100 cos(phi) + noise * N(0,1),
+// phi = (amp0 + amp1 t) peaks(X, Y) + 2 pi carrier x, X/Y = linspace(-3, 3).
+// N(0,1): SplitMix64 chained over (seed, t, pixel) + FP64 Box-Muller (cosine).
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t hash3(uint64_t seed, uint64_t t, uint64_t p) {
+  return mix64(mix64(mix64(seed) ^ t) ^ p);
+}
+__device__ __forceinline__ double peaks(double X, double Y) {
+  return 3.0 * (1.0 - X) * (1.0 - X) * exp(-X * X - (Y + 1.0) * (Y + 1.0)) -
+         10.0 * (X / 5.0 - X * X * X - Y * Y * Y * Y * Y) * exp(-X * X - Y * Y) -
+         exp(-(X + 1.0) * (X + 1.0) - Y * Y) / 3.0;
+}
+
+__global__ __launch_bounds__(256) void gen_frames_kernel(float* __restrict__ out, int W, int H,
+                                                         int t0, int T, uint64_t seed,
+                                                         double amp0, double amp1,
+                                                         double carrier, double noise) {
+  const size_t n = (size_t)W * H;
+  const size_t total = n * T;
+  for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (size_t)gridDim.x * blockDim.x) {
+    const int tt = (int)(g / n);
+    const size_t p = g % n;
+    const int i = (int)(p / W), j = (int)(p % W);
+    const double X = W > 1 ? -3.0 + 6.0 * (double)j / (double)(W - 1) : 3.0;
+    const double Y = H > 1 ? -3.0 + 6.0 * (double)i / (double)(H - 1) : 3.0;
+    const int t = t0 + tt;
+    const double amp = amp0 + amp1 * (double)t;
+    const double ph = amp * peaks(X, Y) + 2.0 * M_PI * carrier * (double)j;
+    const uint64_t h1 = hash3(seed, (uint64_t)t, 2 * p), h2 = hash3(seed, (uint64_t)t, 2 * p + 1);
+    const double u1 = ((double)(h1 >> 11) + 0.5) * 0x1.0p-53;
+    const double u2 = (double)(h2 >> 11) * 0x1.0p-53;
+    const double nz = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+    out[g] = (float)(128.0 + 100.0 * cos(ph) + noise * nz);
+  }
+}
+
+hipError_t launch_gen_frames(float* out, int W, int H, int t0, int T, uint64_t seed, double amp0,
+                             double amp1, double carrier, double noise, hipStream_t s) {
+  hipLaunchKernelGGL(gen_frames_kernel, dim3(4096), dim3(256), 0, s, out, W, H, t0, T, seed, amp0,
+                     amp1, carrier, noise);
+  return hipGetLastError();
+}
+
+}  // namespace glsu

@@ -1,0 +1,79 @@
+// glsu_internal.h -- host-side declarations shared by the library's .hip files.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace glsu {
+
+// ---- FFT plans compiled into the library --------------------------------
+// N = T * E, T lanes per transform, E elements per lane.
+struct FftPlanInfo {
+  int id;
+  int T, E, N;
+  const char* radices;  // cross-lane radices, for reports
+};
+const std::vector<FftPlanInfo>& fft_plans();
+// Plans with N >= need, sorted by N (then by fewest LDS stages).
+std::vector<int> fft_candidates(int need);
+
+// ---- WFF2 device workspace / plan ---------------------------------------
+struct WffPlan {
+  int W = 0, H = 0;
+  int sx = 0, sy = 0;
+  int nwx = 0, nwy = 0;
+  double sigx = 0, sigy = 0, wxl = 0, wxh = 0, wxi = 0, wyl = 0, wyh = 0, wyi = 0;
+  double thr_param = -1;     // < 0: 6*sqrt(mean|f|^2/3) from the first call
+  bool thr_ready = false;
+  float scale = 0;           // wxi*wyi/(4 pi^2)
+  int planx = -1, plany = -1;  // FFT plan ids along x (rows) and y (columns)
+  int Nx = 0, Ny = 0;
+  int cols_waves = 4, rows_waves = 1;  // launch shapes
+  // device buffers
+  float2* d_Hx = nullptr;   // [nwx][Nx], window spectra / Nx
+  float2* d_Hy = nullptr;   // [nwy][Ny], window spectra / Ny
+  float2* d_twx = nullptr;  // W_Nx^k
+  float2* d_twy = nullptr;  // W_Ny^k
+  float2* d_A = nullptr;    // [nwx] tiled planes: row-convolved field
+  float2* d_D = nullptr;    // [nwx] tiled planes: column-filtered field
+  float* d_thr = nullptr;   // threshold (device scalar)
+  float* d_red = nullptr;   // reduction scratch
+  size_t plane_elems = 0;   // elements per tiled plane
+};
+
+constexpr int kTileCols = 8;  // columns per tile in the intermediate planes
+
+int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh, double wxi,
+                  double sigy, double wyl, double wyh, double wyi, double thr, int planx,
+                  int plany, std::string& err);
+void wff_plan_free(WffPlan& p);
+
+// Kernel launchers (all asynchronous on `s`).
+hipError_t launch_thr(const WffPlan& p, const float2* f, hipStream_t s);
+hipError_t launch_rows_fwd(const WffPlan& p, const float2* f, hipStream_t s);
+hipError_t launch_cols(const WffPlan& p, hipStream_t s);
+// mode 0: out = filtered complex field (H*W float2)
+hipError_t launch_rows_inv_filtered(const WffPlan& p, float2* out, hipStream_t s);
+// mode 1: DPRA phase update fused into the inverse row pass
+struct UpdateArgs {
+  const float* phiref;
+  const float* dref_hi;
+  const float* dref_lo;
+  float* phicurr;
+  float* dphi_hi;
+  float* dphi_lo;
+  float* dphi_out;   // may be null
+  float2* filtered;  // may be null
+};
+hipError_t launch_rows_inv_update(const WffPlan& p, const UpdateArgs& u, hipStream_t s);
+
+// DPRA least squares: z = exp(j*Delta) from frame + phiref
+enum InDtype { IN_U8 = 0, IN_F32 = 1, IN_F64 = 2 };
+hipError_t launch_dpra_ls(const void* frame, int dtype, const float* phiref, float2* z, int W,
+                          int H, int pitch, hipStream_t s);
+// synthetic fringe frames (device generator, same definition as the oracle's)
+hipError_t launch_gen_frames(float* out, int W, int H, int t0, int T, uint64_t seed, double amp0,
+                             double amp1, double carrier, double noise, hipStream_t s);
+
+}  // namespace glsu

@@ -1,0 +1,241 @@
+// wff_kernels.h -- WFF2 kernel templates (included by the wff_*.hip units).
+// WFF2 on gfx950: separable windowed-Fourier filtering.
+//
+// Reference semantics (WFT/WFT2_CPU.cpp:190-390):
+//   F = wxi*wyi/(4 pi^2) * sum_{wy,wx} w_p * T( w_p * f ),
+//   w_p(y,x) = g(y,x) exp(j(wx x + wy y)),  T(u) = u if |u| >= thr else 0,
+// with * the linear "same" convolution and T applied on the image only.
+//
+// The window is separable, g = gy(y) gx(x), so w_p = [gy e^{j wy y}] [gx e^{j wx x}]
+// and each 2-D convolution is a convolution along x followed by one along y.
+// The pair loop is reorganised as
+//   A_wx   = f (*)_x hx_wx                         (rows:    1 FFT + nwx IFFTs per row)
+//   D_wx   = sum_wy hy_wy (*)_y T(hy_wy (*)_y A_wx) (columns: 2 + 2 nwy FFTs per column)
+//   F      = c * sum_wx hx_wx (*)_x D_wx            (rows:    nwx FFTs + 1 IFFT per row)
+// All convolutions are circular of length Nx >= W + sx and Ny >= H + sy with a
+// centred kernel, which equals the linear convolution on the image exactly.
+// The whole wy loop of a column runs out of VGPRs and LDS: the column
+// spectrum, its per-pair products and the running sum never leave the CU.
+#pragma once
+#include "fft_device.h"
+#include "glsu_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace glsu {
+// ------------------------------------------------------------ plan table
+#define GLSU_PLANS(X)                    \
+  X(0, 64, 4, 4, 4, 4)                   \
+  X(1, 64, 8, 8, 8)                      \
+  X(2, 64, 12, 4, 4, 4)                  \
+  X(3, 64, 16, 16, 4)                    \
+  X(4, 64, 20, 4, 4, 4)                  \
+  X(5, 64, 24, 8, 8)                     \
+  X(6, 64, 28, 4, 4, 4)                  \
+  X(7, 64, 32, 8, 8)                     \
+  X(8, 64, 36, 4, 4, 4)                  \
+  X(9, 64, 40, 8, 8)                     \
+  X(10, 16, 8, 4, 4)                     \
+  X(11, 16, 12, 4, 4)                    \
+  X(12, 16, 20, 4, 4)                    \
+  X(13, 32, 12, 4, 4, 2)                 \
+  X(14, 32, 20, 4, 4, 2)                 \
+  X(15, 32, 36, 4, 4, 2)
+
+// -------------------------------------------------------- XCD-aware order
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
+// XCD walks a contiguous range of work items (shared L2 lines stay local).
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+  const int q = nblk >> 3, r = nblk & 7;
+  const int k = bid & 7, local = bid >> 3;
+  return k * q + (k < r ? k : r) + local;
+}
+
+__device__ __forceinline__ size_t tile_index(int y, int x, int H) {
+  return ((size_t)(x / kTileCols) * H + y) * kTileCols + (x % kTileCols);
+}
+
+// ---------------------------------------------------------- row forward
+// One transform per row: Zr = FFT_x(f row); for each wx: A_wx row = IFFT_x(Zr * Hx_wx).
+template <class P, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void wff_rows_fwd_kernel(
+    const float2* __restrict__ f, float2* __restrict__ A, const float2* __restrict__ Hx,
+    const float2* __restrict__ tw_g, int W, int H, int nwx, size_t plane) {
+  constexpr int T = P::T, E = P::E, N = P::N;
+  extern __shared__ float2 smem[];
+  float2* tw = smem;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = tw_g[i];
+  __syncthreads();
+  const int slot = threadIdx.x / T, lane = threadIdx.x % T;
+  float2* buf = smem + N + slot * P::LDS;
+  const int y = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / T) + slot;
+  if (y >= H) return;
+  float2 z[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int x = lane + T * j;
+    z[j] = x < W ? f[(size_t)y * W + x] : make_float2(0.f, 0.f);
+  }
+  FftRun<P>::template run<false>(z, buf, tw, lane);
+  for (int k = 0; k < nwx; ++k) {
+    float2 v[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) v[j] = cmul(z[j], Hx[(size_t)k * N + lane + T * j]);
+    FftRun<P>::template run<true>(v, buf, tw, lane);
+    float2* out = A + plane * k;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const int x = lane + T * j;
+      if (x < W) out[tile_index(y, x, H)] = v[j];
+    }
+  }
+}
+
+// -------------------------------------------------------------- columns
+// One transform slot per (wx, column).  Ahat = FFT_y(A col); for each wy:
+// S = IFFT_y(Ahat*Hy); T = threshold(S) on image rows; acc += FFT_y(T)*Hy.
+// D col = IFFT_y(acc).
+template <class P, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
+    const float2* __restrict__ A, float2* __restrict__ D, const float2* __restrict__ Hy,
+    const float2* __restrict__ tw_g, const float* __restrict__ thr_p, int W, int H, int nwx,
+    int nwy, size_t plane) {
+  constexpr int T = P::T, E = P::E, N = P::N;
+  extern __shared__ float2 smem[];
+  float2* tw = smem;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = tw_g[i];
+  __syncthreads();
+  const int slot = threadIdx.x / T, lane = threadIdx.x % T;
+  float2* buf = smem + N + slot * P::LDS;
+  const int item = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / T) + slot;
+  if (item >= nwx * W) return;
+  const int k = item / W, x = item % W;
+  const float thr = *thr_p;
+  const float thr2 = thr * thr;
+  const float2* colA = A + plane * k;
+  float2 ah[E], acc[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int y = lane + T * j;
+    ah[j] = y < H ? colA[tile_index(y, x, H)] : make_float2(0.f, 0.f);
+    acc[j] = make_float2(0.f, 0.f);
+  }
+  FftRun<P>::template run<false>(ah, buf, tw, lane);
+  for (int q = 0; q < nwy; ++q) {
+    const float2* hy = Hy + (size_t)q * N;
+    float2 v[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) v[j] = cmul(ah[j], hy[lane + T * j]);
+    FftRun<P>::template run<true>(v, buf, tw, lane);
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const int y = lane + T * j;
+      const float m2 = v[j].x * v[j].x + v[j].y * v[j].y;
+      if (!(y < H && m2 >= thr2)) v[j] = make_float2(0.f, 0.f);
+    }
+    FftRun<P>::template run<false>(v, buf, tw, lane);
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const float2 h = hy[lane + T * j];
+      acc[j].x += v[j].x * h.x - v[j].y * h.y;
+      acc[j].y += v[j].x * h.y + v[j].y * h.x;
+    }
+  }
+  FftRun<P>::template run<true>(acc, buf, tw, lane);
+  float2* colD = D + plane * k;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int y = lane + T * j;
+    if (y < H) colD[tile_index(y, x, H)] = acc[j];
+  }
+}
+
+// ---------------------------------------------------------- row inverse
+// acc = sum_wx FFT_x(D_wx row) * Hx_wx ; F = scale * IFFT_x(acc)
+// MODE 0: write F.  MODE 1: DPRA phase update (DPRA/dpra_cpu.cpp:244-251):
+//   t = angle(F); phicurr = wrap(phiref + t); dphi = dphiref + t (two-float sum).
+template <class P, int WAVES, int MODE>
+__global__ __launch_bounds__(64 * WAVES) void wff_rows_inv_kernel(
+    const float2* __restrict__ D, const float2* __restrict__ Hx, const float2* __restrict__ tw_g,
+    int W, int H, int nwx, size_t plane, float scale, float2* __restrict__ filtered,
+    UpdateArgs u) {
+  constexpr int T = P::T, E = P::E, N = P::N;
+  extern __shared__ float2 smem[];
+  float2* tw = smem;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = tw_g[i];
+  __syncthreads();
+  const int slot = threadIdx.x / T, lane = threadIdx.x % T;
+  float2* buf = smem + N + slot * P::LDS;
+  const int y = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / T) + slot;
+  if (y >= H) return;
+  float2 acc[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) acc[j] = make_float2(0.f, 0.f);
+  for (int k = 0; k < nwx; ++k) {
+    const float2* Dk = D + plane * k;
+    float2 v[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const int x = lane + T * j;
+      v[j] = x < W ? Dk[tile_index(y, x, H)] : make_float2(0.f, 0.f);
+    }
+    FftRun<P>::template run<false>(v, buf, tw, lane);
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const float2 h = Hx[(size_t)k * N + lane + T * j];
+      acc[j].x += v[j].x * h.x - v[j].y * h.y;
+      acc[j].y += v[j].x * h.y + v[j].y * h.x;
+    }
+  }
+  FftRun<P>::template run<true>(acc, buf, tw, lane);
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int x = lane + T * j;
+    if (x >= W) continue;
+    const size_t id = (size_t)y * W + x;
+    const float2 F = make_float2(acc[j].x * scale, acc[j].y * scale);
+    if constexpr (MODE == 0) {
+      filtered[id] = F;
+    } else {
+      if (u.filtered) u.filtered[id] = F;
+      const float t = atan2f(F.y, F.x);
+      // wrap(phiref + t) into (-pi, pi]; 2*pi split in two floats
+      float pc = u.phiref[id] + t;
+      if (pc > 3.14159265f) pc = (pc - 6.28318548f) + 1.74845553e-7f;
+      else if (pc <= -3.14159265f) pc = (pc + 6.28318548f) - 1.74845553e-7f;
+      u.phicurr[id] = pc;
+      // dphi = dphiref + t, carried as an unevaluated float pair (TwoSum)
+      const float hi = u.dref_hi[id], lo = u.dref_lo[id];
+      const float s = hi + t;
+      const float bp = s - hi;
+      const float err = (hi - (s - bp)) + (t - bp);
+      const float lo2 = lo + err;
+      const float nh = s + lo2;                // renormalise (Fast2Sum)
+      const float nl = lo2 - (nh - s);
+      u.dphi_hi[id] = nh;
+      u.dphi_lo[id] = nl;
+      if (u.dphi_out) u.dphi_out[id] = nh;
+    }
+  }
+}
+
+// ============================================================ host side
+static int plan_smem_bytes(int id, int waves) {
+  for (auto& p : fft_plans())
+    if (p.id == id) return (int)sizeof(float2) * (p.N + (waves * 64 / p.T) * (p.N + p.N / p.E));
+  return 0;
+}
+
+template <class P>
+static int items_per_block(int waves) {
+  return waves * 64 / P::T;
+}
+
+#define GLSU_DISPATCH_CASE(id, T, E, ...) \
+  case id: {                              \
+    using P = FftPlan<T, E, __VA_ARGS__>; \
+    BODY;                                 \
+  } break;
+
+}  // namespace glsu

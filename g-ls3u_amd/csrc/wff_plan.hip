@@ -1,0 +1,203 @@
+// wff_plan.hip -- WFF2 plan set-up (window spectra, twiddles, buffers) and threshold.
+#include "wff_kernels.h"
+
+namespace glsu {
+
+#define GLSU_PLAN_INFO(id, T, E, ...) {id, T, E, T * E, #__VA_ARGS__},
+const std::vector<FftPlanInfo>& fft_plans() {
+  static const std::vector<FftPlanInfo> v = {GLSU_PLANS(GLSU_PLAN_INFO)};
+  return v;
+}
+
+std::vector<int> fft_candidates(int need) {
+  std::vector<std::pair<int, int>> c;
+  for (auto& p : fft_plans())
+    if (p.N >= need) c.push_back({p.N, p.id});
+  std::sort(c.begin(), c.end());
+  std::vector<int> out;
+  for (auto& q : c) out.push_back(q.second);
+  return out;
+}
+
+// ------------------------------------------------------------ threshold
+// thr = 6 * sqrt(sum |f|^2 / (W*H) / 3), WFT/WFT2_CPU.cpp:982-1016.
+__global__ __launch_bounds__(256) void thr_partial_kernel(const float2* __restrict__ f, size_t n,
+                                                          float* __restrict__ part) {
+  float s = 0.f;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const float2 v = f[i];
+    s += v.x * v.x + v.y * v.y;
+  }
+  __shared__ float red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void thr_final_kernel(const float* __restrict__ part, int nparts,
+                                                        float inv_n, float* __restrict__ thr) {
+  __shared__ float red[256];
+  float s = 0.f;
+  for (int i = threadIdx.x; i < nparts; i += 256) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *thr = 6.f * sqrtf(red[0] * inv_n / 3.f);
+}
+
+hipError_t launch_thr(const WffPlan& p, const float2* f, hipStream_t s) {
+  const size_t n = (size_t)p.W * p.H;
+  const int parts = 256;
+  hipLaunchKernelGGL(thr_partial_kernel, dim3(parts), dim3(256), 0, s, f, n, p.d_red);
+  hipLaunchKernelGGL(thr_final_kernel, dim3(1), dim3(256), 0, s, p.d_red, parts,
+                     (float)(1.0 / (double)n), p.d_thr);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------- plan set-up
+static const FftPlanInfo* find_plan(int id) {
+  for (auto& p : fft_plans())
+    if (p.id == id) return &p;
+  return nullptr;
+}
+
+// (1/N) * DFT of the centred, modulated 1-D window g1(d) e^{j w d}, d in [-s, s]:
+// H[k] = (1/N) sum_d g1(d) e^{j w d} e^{-2 pi i k d / N}
+static std::vector<float2> window_spectrum(const std::vector<double>& g1, int s, double w, int N) {
+  std::vector<float2> out(N);
+  for (int k = 0; k < N; ++k) {
+    double re = 0, im = 0;
+    for (int d = -s; d <= s; ++d) {
+      long kd = ((long)k * d) % N;
+      if (kd < 0) kd += N;
+      const double a = w * d - 2.0 * M_PI * (double)kd / (double)N;
+      const double g = g1[d + s];
+      re += g * std::cos(a);
+      im += g * std::sin(a);
+    }
+    out[k] = make_float2((float)(re / N), (float)(im / N));
+  }
+  return out;
+}
+
+static std::vector<double> norm_window(double sig, int s) {
+  // exp(-d^2 / 2 / sigma^2) scaled to unit discrete L2 norm (WFT/WFT2_CPU.cpp:752-805;
+  // the 2-D norm is the product of the 1-D norms)
+  std::vector<double> g(2 * s + 1);
+  double n2 = 0;
+  for (int d = -s; d <= s; ++d) {
+    g[d + s] = std::exp(-(double)(d * d) / 2 / sig / sig);
+    n2 += g[d + s] * g[d + s];
+  }
+  const double inv = 1.0 / std::sqrt(n2);
+  for (auto& v : g) v *= inv;
+  return g;
+}
+
+static std::vector<float2> twiddles(int N) {
+  std::vector<float2> t(N);
+  for (int k = 0; k < N; ++k) {
+    const double a = 2.0 * M_PI * (double)k / (double)N;
+    t[k] = make_float2((float)std::cos(a), (float)-std::sin(a));
+  }
+  return t;
+}
+
+template <class V>
+static hipError_t upload(V*& dptr, const std::vector<V>& h) {
+  hipError_t e = hipMalloc((void**)&dptr, sizeof(V) * h.size());
+  if (e != hipSuccess) return e;
+  return hipMemcpy(dptr, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice);
+}
+
+int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh, double wxi,
+                  double sigy, double wyl, double wyh, double wyi, double thr, int planx,
+                  int plany, std::string& err) {
+  p = WffPlan{};
+  p.W = W;
+  p.H = H;
+  p.sigx = sigx; p.sigy = sigy;
+  p.wxl = wxl; p.wxh = wxh; p.wxi = wxi;
+  p.wyl = wyl; p.wyh = wyh; p.wyi = wyi;
+  p.thr_param = thr;
+  // WFT/WFT2_CPU.cpp:701-702, :222-223
+  p.sx = (int)std::round(3 * sigx);
+  p.sy = (int)std::round(3 * sigy);
+  p.nwx = (int)((wxh - wxl) * (1 / wxi)) + 1;
+  p.nwy = (int)((wyh - wyl) * (1 / wyi)) + 1;
+  if (p.nwx < 1 || p.nwy < 1) {
+    err = "empty frequency grid";
+    return -1;
+  }
+  p.scale = (float)(0.25 * (1.0 / (M_PI * M_PI)) * wxi * wyi);
+  const int needx = W + p.sx, needy = H + p.sy;
+  if (planx < 0) {
+    auto c = fft_candidates(needx);
+    if (c.empty()) { err = "no FFT plan long enough along x"; return -2; }
+    planx = c[0];
+  }
+  if (plany < 0) {
+    auto c = fft_candidates(needy);
+    if (c.empty()) { err = "no FFT plan long enough along y"; return -2; }
+    plany = c[0];
+  }
+  const FftPlanInfo* px = find_plan(planx);
+  const FftPlanInfo* py = find_plan(plany);
+  if (!px || !py || px->N < needx || py->N < needy) {
+    err = "FFT plan too short for image + half window";
+    return -2;
+  }
+  p.planx = planx; p.plany = plany;
+  p.Nx = px->N; p.Ny = py->N;
+  // keep dynamic LDS per workgroup within 64 KiB
+  while (p.cols_waves > 1 && plan_smem_bytes(plany, p.cols_waves) > 65536) p.cols_waves /= 2;
+  while (p.rows_waves > 1 && plan_smem_bytes(planx, p.rows_waves) > 65536) p.rows_waves /= 2;
+  const int wb = (W + kTileCols - 1) / kTileCols;
+  p.plane_elems = (size_t)wb * kTileCols * H;
+  // spectra tables
+  auto gx = norm_window(sigx, p.sx), gy = norm_window(sigy, p.sy);
+  std::vector<float2> hx, hy;
+  for (int k = 0; k < p.nwx; ++k) {
+    auto t = window_spectrum(gx, p.sx, wxl + (double)k * wxi, p.Nx);
+    hx.insert(hx.end(), t.begin(), t.end());
+  }
+  for (int k = 0; k < p.nwy; ++k) {
+    auto t = window_spectrum(gy, p.sy, wyl + (double)k * wyi, p.Ny);
+    hy.insert(hy.end(), t.begin(), t.end());
+  }
+  hipError_t e;
+  if ((e = upload(p.d_Hx, hx)) != hipSuccess || (e = upload(p.d_Hy, hy)) != hipSuccess ||
+      (e = upload(p.d_twx, twiddles(p.Nx))) != hipSuccess ||
+      (e = upload(p.d_twy, twiddles(p.Ny))) != hipSuccess ||
+      (e = hipMalloc((void**)&p.d_A, sizeof(float2) * p.plane_elems * p.nwx)) != hipSuccess ||
+      (e = hipMalloc((void**)&p.d_D, sizeof(float2) * p.plane_elems * p.nwx)) != hipSuccess ||
+      (e = hipMalloc((void**)&p.d_thr, sizeof(float))) != hipSuccess ||
+      (e = hipMalloc((void**)&p.d_red, sizeof(float) * 256)) != hipSuccess) {
+    err = std::string("device allocation failed: ") + hipGetErrorString(e);
+    wff_plan_free(p);
+    return -4;
+  }
+  if (thr >= 0) {
+    const float t = (float)thr;
+    hipMemcpy(p.d_thr, &t, sizeof(float), hipMemcpyHostToDevice);
+    p.thr_ready = true;
+  }
+  return 0;
+}
+
+void wff_plan_free(WffPlan& p) {
+  hipFree(p.d_Hx); hipFree(p.d_Hy); hipFree(p.d_twx); hipFree(p.d_twy);
+  hipFree(p.d_A); hipFree(p.d_D); hipFree(p.d_thr); hipFree(p.d_red);
+  p.d_Hx = p.d_Hy = p.d_twx = p.d_twy = p.d_A = p.d_D = nullptr;
+  p.d_thr = p.d_red = nullptr;
+}
+
+}  // namespace glsu

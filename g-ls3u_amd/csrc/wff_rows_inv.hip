@@ -1,0 +1,42 @@
+// wff_rows_inv.hip -- launchers of the row-inverse WFF2 kernel (filtered / DPRA update).
+#include "wff_kernels.h"
+
+namespace glsu {
+
+template <int MODE>
+static hipError_t launch_rows_inv_impl(const WffPlan& p, float2* out, const UpdateArgs& u,
+                                       hipStream_t s) {
+  const int waves = p.rows_waves;
+  const int smem = plan_smem_bytes(p.planx, waves);
+  switch (p.planx) {
+#define BODY                                                                                   \
+  {                                                                                            \
+    const int per = items_per_block<P>(waves);                                                 \
+    const int grid = (p.H + per - 1) / per;                                                    \
+    if (waves == 1)                                                                            \
+      hipLaunchKernelGGL((wff_rows_inv_kernel<P, 1, MODE>), dim3(grid), dim3(64), smem, s,     \
+                         p.d_D, p.d_Hx, p.d_twx, p.W, p.H, p.nwx, p.plane_elems, p.scale, out, \
+                         u);                                                                   \
+    else                                                                                       \
+      hipLaunchKernelGGL((wff_rows_inv_kernel<P, 2, MODE>), dim3(grid), dim3(128), smem, s,    \
+                         p.d_D, p.d_Hx, p.d_twx, p.W, p.H, p.nwx, p.plane_elems, p.scale, out, \
+                         u);                                                                   \
+  }
+    GLSU_PLANS(GLSU_DISPATCH_CASE)
+#undef BODY
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_rows_inv_filtered(const WffPlan& p, float2* out, hipStream_t s) {
+  UpdateArgs u{};
+  return launch_rows_inv_impl<0>(p, out, u, s);
+}
+
+hipError_t launch_rows_inv_update(const WffPlan& p, const UpdateArgs& u, hipStream_t s) {
+  return launch_rows_inv_impl<1>(p, nullptr, u, s);
+}
+
+}  // namespace glsu

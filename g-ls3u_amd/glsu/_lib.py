@@ -1,0 +1,131 @@
+"""ctypes binding of libglsu.so (include/glsu.h).
+
+The library is the product: HIP kernels for gfx950 behind a C ABI.  There is
+no fallback -- if the shared object is missing or no GPU is visible, calls
+raise.
+"""
+import ctypes
+import os
+import subprocess
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))  # g-ls3u_amd/
+LIB_PATH = os.path.join(_PKG_ROOT, "lib", "libglsu.so")
+CSRC = os.path.join(_PKG_ROOT, "csrc")
+
+OK = 0
+ERR_ARG = -1
+ERR_PADDING = -2
+ERR_SIZE = -3
+ERR_DEVICE = -4
+ERR_UNSUPPORTED = -5
+
+WFF = 0
+WFR = 1
+U8 = 0
+F32 = 1
+F64 = 2
+
+
+class GlsuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"glsu error {code}: {msg}")
+        self.code = code
+
+
+class PaddingError(GlsuError):
+    """Padded FFT size out of range; the reference constructors throw int -1."""
+
+
+class WffParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("sigma_x", "wxl", "wxh", "wxi", "sigma_y", "wyl", "wyh", "wyi", "thr")]
+
+
+_lib = None
+
+
+def build(jobs=8):
+    """Compile libglsu.so for gfx950 (hipcc; runs without a GPU)."""
+    subprocess.run(["make", "-s", "-C", CSRC, f"-j{jobs}"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libglsu.so not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    I, D, V = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    PI, PD = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+    PV = ctypes.POINTER(ctypes.c_void_p)
+    PP = ctypes.POINTER(WffParams)
+    PF = ctypes.c_void_p
+    U64 = ctypes.c_uint64
+    sig = {
+        "glsu_last_error": (ctypes.c_char_p, []),
+        "glsu_version": (ctypes.c_char_p, []),
+        "glsu_device_count": (I, [PI]),
+        "glsu_wff_params_default": (I, [PP]),
+        "glsu_wff_params_dpra_reference": (I, [PP]),
+        "glsu_wft2_create": (I, [PV, I, I, I, PP, V]),
+        "glsu_wft2_create_plan": (I, [PV, I, I, I, PP, V, I, I]),
+        "glsu_wft2_compute": (I, [V, V, V, PD]),
+        "glsu_wft2_compute_host": (I, [V, PF, PF, PD]),
+        "glsu_wft2_info": (I, [V, PI, PI, PI, PI, PI, PI, PD]),
+        "glsu_wft2_destroy": (I, [V]),
+        "glsu_dpra_create": (I, [PV, PD, I, I, I, PP, V]),
+        "glsu_dpra_create_f32": (I, [PV, PF, I, I, I, PP, V]),
+        "glsu_dpra_create_plan": (I, [PV, PD, I, I, I, PP, V, I, I]),
+        "glsu_dpra_frame": (I, [V, V, I, V, V, PD]),
+        "glsu_dpra_frame_host": (I, [V, PF, I, PF, PD]),
+        "glsu_dpra_update_ref": (I, [V]),
+        "glsu_dpra_run": (I, [V, V, I, I, V, PD]),
+        "glsu_dpra_run_graph": (I, [V, V, I, I, V, PD]),
+        "glsu_dpra_profile": (I, [V, V, I, I, V, PD]),
+        "glsu_dpra_get_state": (I, [V, PF, PF]),
+        "glsu_dpra_set_state": (I, [V, PF, PF]),
+        "glsu_dpra_copy_z": (I, [V, V]),
+        "glsu_dpra_thr": (I, [V, PD]),
+        "glsu_dpra_destroy": (I, [V]),
+        "glsu_gen_frames": (I, [V, I, I, I, I, U64, D, D, D, D, V]),
+        "glsu_fft_plans": (I, [PI, PI, PI, PI, I, PI]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != OK:
+        msg = lib().glsu_last_error().decode(errors="replace")
+        if rc == ERR_PADDING:
+            raise PaddingError(rc, msg)
+        raise GlsuError(rc, msg)
+    return rc
+
+
+def exported_symbols():
+    """Names declared in include/glsu.h (for the load/export test)."""
+    return [
+        "glsu_last_error", "glsu_version", "glsu_device_count", "glsu_wff_params_default",
+        "glsu_wff_params_dpra_reference", "glsu_wft2_create", "glsu_wft2_create_plan",
+        "glsu_wft2_compute", "glsu_wft2_compute_host", "glsu_wft2_info", "glsu_wft2_destroy",
+        "glsu_dpra_create", "glsu_dpra_create_f32", "glsu_dpra_create_plan", "glsu_dpra_frame",
+        "glsu_dpra_frame_host", "glsu_dpra_update_ref", "glsu_dpra_run", "glsu_dpra_run_graph",
+        "glsu_dpra_profile", "glsu_dpra_get_state", "glsu_dpra_set_state", "glsu_dpra_copy_z", "glsu_dpra_thr",
+        "glsu_dpra_destroy", "glsu_gen_frames", "glsu_fft_plans",
+    ]
+
+
+def fft_plans():
+    n = ctypes.c_int()
+    ids = (ctypes.c_int * 64)()
+    t = (ctypes.c_int * 64)()
+    e = (ctypes.c_int * 64)()
+    ln = (ctypes.c_int * 64)()
+    check(lib().glsu_fft_plans(ids, t, e, ln, 64, ctypes.byref(n)))
+    return [dict(id=ids[i], T=t[i], E=e[i], N=ln[i]) for i in range(n.value)]

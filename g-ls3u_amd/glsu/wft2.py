@@ -1,0 +1,66 @@
+"""WFT2 (WFF mode) on the GPU -- mirror of WFT_FPA::WFT::WFT2_CUDAF.
+
+Reference interface: WFT/WFT2_CUDAf.h:21-40 (ctor(W, H, type, sigma_x, wxl,
+wxh, wxi, sigma_y, wyl, wyh, wyi, thr, z, nStreams); operator()(d_f, d_z, time)).
+Results live in device memory, as WFT2_DeviceResultsF::m_d_filtered does.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import WFF, check, lib
+from .params import wff_params
+
+
+class WFT2:
+    def __init__(self, width, height, type=WFF, params=None, stream=None, planx=-1, plany=-1):
+        """params: WffParams (see glsu.wff_params); None -> wff_params() defaults
+        of the configs (sigma 10, [-0.2, 0.2] step 0.1, thr < 0)."""
+        self.width, self.height = int(width), int(height)
+        self.params = params if params is not None else wff_params()
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        h = ctypes.c_void_p()
+        check(lib().glsu_wft2_create_plan(ctypes.byref(h), self.width, self.height, int(type),
+                                          ctypes.byref(self.params),
+                                          ctypes.c_void_p(self.stream.cuda_stream),
+                                          int(planx), int(plany)))
+        self._h = h
+        self.last_ms = None
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._h = None
+            try:
+                self._destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
+
+    @staticmethod
+    def _destroy(h):
+        lib().glsu_wft2_destroy(h)
+
+    __del__ = close
+
+    def info(self):
+        v = [ctypes.c_int() for _ in range(6)]
+        thr = ctypes.c_double()
+        check(lib().glsu_wft2_info(self._h, *[ctypes.byref(x) for x in v], ctypes.byref(thr)))
+        nx, ny, sx, sy, nwx, nwy = (x.value for x in v)
+        return dict(nx=nx, ny=ny, sx=sx, sy=sy, nwx=nwx, nwy=nwy, thr=thr.value)
+
+    def __call__(self, f, out=None, timed=False):
+        """f: complex64 CUDA tensor (height, width).  Returns the filtered field."""
+        if f.dtype != torch.complex64 or not f.is_cuda or not f.is_contiguous():
+            raise ValueError("f must be a contiguous complex64 CUDA tensor")
+        if tuple(f.shape) != (self.height, self.width):
+            raise ValueError(f"f has shape {tuple(f.shape)}, expected {(self.height, self.width)}")
+        if out is None:
+            out = torch.empty_like(f)
+        ms = ctypes.c_double()
+        check(lib().glsu_wft2_compute(self._h, ctypes.c_void_p(f.data_ptr()),
+                                      ctypes.c_void_p(out.data_ptr()),
+                                      ctypes.byref(ms) if timed else None))
+        self.last_ms = ms.value if timed else None
+        return out

@@ -1,0 +1,169 @@
+/*
+ * glsu.h -- C ABI of the MI355X (gfx950) G-LS3U hot path: DPRA least squares
+ * + WFF2 windowed Fourier filtering + reference-phase update.
+ *
+ * Every entry point replaces a member of the reference's C++ classes (cited
+ * per function, paths relative to the TWANG006/G-LS3U tree).  Plain pointers and
+ * sizes only; device pointers are HIP device addresses, host pointers are
+ * ordinary host memory.  All calls return GLSU_OK (0) or a negative status;
+ * glsu_last_error() gives the message of the last failure on the calling
+ * thread.  A handle is not re-entrant; different handles are independent.  All
+ * work of a handle is ordered on its stream: the one passed at create time, or
+ * the default (null) stream when NULL is passed, as in the reference's GPU
+ * classes.
+ */
+#ifndef GLSU_H
+#define GLSU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GLSU_OK 0
+#define GLSU_ERR_ARG (-1)         /* invalid argument / null handle                 */
+#define GLSU_ERR_PADDING (-2)     /* padded FFT size out of range (ref: throw -1)    */
+#define GLSU_ERR_SIZE (-3)        /* frame size differs from the initialised size    */
+#define GLSU_ERR_DEVICE (-4)      /* HIP runtime error                               */
+#define GLSU_ERR_UNSUPPORTED (-5) /* mode not provided by this library (WFR2)        */
+
+/* WFT_FPA::WFT::WFT_TYPE, Utils/WFT-FPA.h:72-76 */
+#define GLSU_WFF 0
+#define GLSU_WFR 1
+
+/* frame element types accepted by the DPRA entry points */
+#define GLSU_U8 0
+#define GLSU_F32 1
+#define GLSU_F64 2
+
+/* WFF parameters of the WFT2 constructors, WFT/WFT2_CPU.h:30-36
+ * (sigma, low, HIGH, step per axis, then the threshold; thr < 0 selects
+ * 6*sqrt(mean|f|^2/3) computed on the first call and cached). */
+typedef struct glsu_wff_params {
+  double sigma_x, wxl, wxh, wxi;
+  double sigma_y, wyl, wyh, wyi;
+  double thr;
+} glsu_wff_params;
+
+typedef struct glsu_wft2 glsu_wft2;
+typedef struct glsu_dpra glsu_dpra;
+
+const char *glsu_last_error(void);
+/* library version string */
+const char *glsu_version(void);
+/* number of visible HIP devices (0 when none) */
+int glsu_device_count(int *count);
+
+/* Defaults of the WFF-mode constructor WFT2_cpu(W, H, WFF, z, n),
+ * WFT/WFT2_CPU.cpp:43-54 (sigma 10, [-2-3/sigma, 2+3/sigma] step 1/sigma, thr<0). */
+int glsu_wff_params_default(glsu_wff_params *p);
+/* Parameters the reference DPRA constructors hard-code,
+ * DPRA/dpra_cpu.cpp:23-25 and DPRA/dpra_cudaf.cu:116-118
+ * (sigma 20, [-0.15f, 0.15f] step 0.05f, thr 15). */
+int glsu_wff_params_dpra_reference(glsu_wff_params *p);
+
+/* ---- WFT2 (WFF mode) --------------------------------------------------------
+ * Replaces WFT2_CUDAF::WFT2_CUDAF(W, H, type, sx, wxl, wxh, wxi, sy, wyl, wyh,
+ * wyi, thr, z, nStreams) + cuWFT2_Initialize + cuWFF2_Init,
+ * WFT/WFT2_CUDAf.h:27-33, WFT/WFT2_CUDAf.cu:1321-1419 (and the FP64 CPU twin
+ * WFT2_cpu, WFT/WFT2_CPU.h:30-36).  Returns GLSU_ERR_PADDING where the
+ * reference constructor throws int -1 (padded size above 4096,
+ * WFT/WFT2_CPU.cpp:69-73), GLSU_ERR_UNSUPPORTED for type == GLSU_WFR. */
+int glsu_wft2_create(glsu_wft2 **out, int width, int height, int type,
+                     const glsu_wff_params *p, void *hip_stream);
+/* Same, with explicit FFT plan ids along x and y (-1 = automatic); used by the
+ * autotuner.  No counterpart in the reference. */
+int glsu_wft2_create_plan(glsu_wft2 **out, int width, int height, int type,
+                          const glsu_wff_params *p, void *hip_stream, int planx,
+                          int plany);
+/* Replaces WFT2_CUDAF::operator()(cufftComplex *d_f, WFT2_DeviceResultsF &d_z,
+ * double &time), WFT/WFT2_CUDAf.h:38-40 -> cuWFF2, WFT/WFT2_CUDAf.cu:1016-1173.
+ * d_f, d_filtered: device arrays of height*width complex64 (interleaved
+ * re,im; row-major).  ms: if non-null, the call synchronises and returns the
+ * elapsed device time in milliseconds; if null the call is asynchronous. */
+int glsu_wft2_compute(glsu_wft2 *h, const void *d_f, void *d_filtered, double *ms);
+/* Host-pointer form of the FP64/FP32 CPU functor operator(), WFT/WFT2_CPU.h:41-43:
+ * copies f in, filters, copies the result out (PCIe-inclusive). */
+int glsu_wft2_compute_host(glsu_wft2 *h, const float *h_f, float *h_filtered, double *ms);
+/* info: padded FFT lengths (this library's, not the reference's OPT sizes),
+ * half window sizes, grid sizes, and the threshold in effect (NaN until the
+ * first call when thr < 0). */
+int glsu_wft2_info(glsu_wft2 *h, int *nx, int *ny, int *sx, int *sy, int *nwx, int *nwy,
+                   double *thr);
+int glsu_wft2_destroy(glsu_wft2 *h);
+
+/* ---- DPRA ------------------------------------------------------------------
+ * Replaces DPRA_CUDAF::DPRA_CUDAF(const float *Phi0, W, H, rr),
+ * DPRA/dpra_cudaf.h:28-30, DPRA/dpra_cudaf.cu:96-164, and the FP64 CPU
+ * DPRA_CPU(const double *Phi0, W, H, rr, nThreads), DPRA/dpra_cpu.h:30-33.
+ * phi0: HOST array of height*width initial phase (row-major).  The WFF
+ * parameters the reference hard-codes are exposed here; p == NULL selects the
+ * reference DPRA's own (glsu_wff_params_dpra_reference).  rr: reference update
+ * rate (>= 1). */
+int glsu_dpra_create(glsu_dpra **out, const double *phi0, int width, int height, int rr,
+                     const glsu_wff_params *p, void *hip_stream);
+int glsu_dpra_create_f32(glsu_dpra **out, const float *phi0, int width, int height, int rr,
+                         const glsu_wff_params *p, void *hip_stream);
+/* Same, with explicit FFT plan ids (-1 = automatic).  No reference counterpart. */
+int glsu_dpra_create_plan(glsu_dpra **out, const double *phi0, int width, int height, int rr,
+                          const glsu_wff_params *p, void *hip_stream, int planx, int plany);
+/* Replaces DPRA_CUDAF::dpra_per_frame(const cv::Mat &img, std::vector<float> &dPhi,
+ * double &time), DPRA/dpra_cudaf.h:41-43, DPRA/dpra_cudaf.cu:207-284 (FP64 twin
+ * DPRA/dpra_cpu.cpp:130-252).  d_frame: DEVICE array of height*width elements of
+ * frame_dtype (GLSU_U8/F32/F64).  d_dphi: device float[height*width] receiving
+ * the phase change w.r.t. frame 0 (may be NULL).  d_phicurr: device float
+ * [height*width] receiving the wrapped current phase (may be NULL).  ms as in
+ * glsu_wft2_compute. */
+int glsu_dpra_frame(glsu_dpra *h, const void *d_frame, int frame_dtype, float *d_dphi,
+                    float *d_phicurr, double *ms);
+/* Host-pointer form, as the reference (memcpy to pinned, H2D, compute, D2H). */
+int glsu_dpra_frame_host(glsu_dpra *h, const void *h_frame, int frame_dtype, float *h_dphi,
+                         double *ms);
+/* Replaces DPRA_CUDAF::update_ref_phi(), DPRA/dpra_cudaf.cu:286-290
+ * (phiref <- phicurr, dphiref <- dphi). */
+int glsu_dpra_update_ref(glsu_dpra *h);
+/* Replaces DPRA_CPU::operator()(frames, dPhi_Sum, time), DPRA/dpra_cpu.cpp:58-89:
+ * for i in [0, nframes): dpra_per_frame(frame i); if (i % rr == 0) update_ref.
+ * d_frames: device, nframes*height*width elements; d_dphi_all: device float
+ * [nframes*height*width] (may be NULL).  The reference's CUDA overloads of
+ * operator() are empty stubs (DPRA/dpra_cudaf.cu:193-205). */
+int glsu_dpra_run(glsu_dpra *h, const void *d_frames, int nframes, int frame_dtype,
+                  float *d_dphi_all, double *ms);
+/* Same chain replayed from captured HIP graphs (one per reference-buffer
+ * parity).  Results equal glsu_dpra_run bit for bit.  No reference counterpart. */
+int glsu_dpra_run_graph(glsu_dpra *h, const void *d_frames, int nframes, int frame_dtype,
+                        float *d_dphi_all, double *ms);
+/* Diagnostics: same chain as glsu_dpra_run with HIP events between the kernels
+ * of every frame; ms[0..3] receive the summed device times (ms) of the
+ * least-squares, row-forward, column and row-inverse+update kernels.
+ * No reference counterpart (the reference prints per-step event times,
+ * DPRA/dpra_cudaf.cu:219-283). */
+int glsu_dpra_profile(glsu_dpra *h, const void *d_frames, int nframes, int frame_dtype,
+                      float *d_dphi_all, double *ms);
+/* State access for checkpoint/resume (the reference cannot seed dphiref,
+ * DPRA/dpra_cpu.cpp:19).  Host float arrays of height*width; any may be NULL. */
+int glsu_dpra_get_state(glsu_dpra *h, float *phiref, float *dphiref);
+int glsu_dpra_set_state(glsu_dpra *h, const float *phiref, const float *dphiref);
+/* copy the last pre-filter field z = exp(j*Delta) (complex64, height*width)
+ * into the device array d_z, ordered on the handle's stream */
+int glsu_dpra_copy_z(glsu_dpra *h, void *d_z);
+/* the WFF threshold in effect (NaN before the first frame when thr < 0) */
+int glsu_dpra_thr(glsu_dpra *h, double *thr);
+int glsu_dpra_destroy(glsu_dpra *h);
+
+/* ---- utilities ---------------------------------------------------------- */
+/* Seeded synthetic fringe frames, written to device memory (float32,
+ * nframes*height*width): I = 128 + 100 cos(phi) + noise*N(0,1),
+ * phi = (amp0 + amp1*t) peaks(X,Y) + 2*pi*carrier*x.  No reference counterpart
+ * (the reference reads image files, App_DPRA/dprawidget.cpp:243-255). */
+int glsu_gen_frames(float *d_out, int width, int height, int t0, int nframes, uint64_t seed,
+                    double amp0, double amp1, double carrier, double noise, void *hip_stream);
+/* FFT plans compiled into the library: fills up to cap entries of
+ * (id, T, E, N) and returns the count in *count. */
+int glsu_fft_plans(int *ids, int *lanes, int *elems, int *lengths, int cap, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GLSU_H */

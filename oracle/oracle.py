@@ -1,0 +1,267 @@
+"""ctypes wrapper around oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+The FP64 CPU oracle restates the reference's CPU path (DPRA/dpra_cpu.cpp +
+WFT/WFT2_CPU.cpp).  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module, and only as the checker / CPU baseline.
+The product (libglsu) never routes through it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build():
+    """Compile liboracle.so from oracle/glsu_oracle.c (gcc, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        D, I, U64 = ctypes.c_double, ctypes.c_int, ctypes.c_uint64
+        V = ctypes.c_void_p
+        sig = {
+            "orc_opt_table": (I, [_ip, I]),
+            "orc_first_greater": (I, [I]),
+            "orc_opt_size": (I, [I]),
+            "orc_pair_count": (I, [D, D, D]),
+            "orc_fft1d": (I, [I, _dp, I]),
+            "orc_fft2d": (I, [I, I, _dp, I]),
+            "orc_hash3": (U64, [U64, U64, U64]),
+            "orc_normal": (D, [U64, U64, U64]),
+            "orc_peaks": (D, [D, D]),
+            "orc_gen_phase": (None, [I, I, D, D, _dp]),
+            "orc_gen_frame": (None, [I, I, I, U64, D, D, D, D, _dp, _dp]),
+            "orc_wft2_create": (V, [I, I, D, D, D, D, D, D, D, D, D, I, I]),
+            "orc_wft2_destroy": (None, [V]),
+            "orc_wft2_padded": (I, [V, _ip, _ip]),
+            "orc_wft2_halfwin": (I, [V, _ip, _ip]),
+            "orc_wft2_pairs": (I, [V, _ip, _ip]),
+            "orc_wft2_thr": (D, [V]),
+            "orc_wft2_compute": (I, [V, _dp, _dp]),
+            "orc_dpra_create": (V, [_dp, I, I, I, D, D, D, D, D, D, D, D, D, I, I]),
+            "orc_dpra_destroy": (None, [V]),
+            "orc_dpra_ls": (I, [V, _dp, _dp]),
+            "orc_dpra_frame": (I, [V, _dp, _dp, _dp]),
+            "orc_dpra_update_ref": (None, [V]),
+            "orc_dpra_run": (I, [V, _dp, I, _dp]),
+            "orc_solve3_sym": (I, [_dp, _dp]),
+            "orc_dpra_thr": (D, [V]),
+            "orc_wrapped_err": (None, [_dp, _dp, ctypes.c_int64, _dp, _dp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# --------------------------------------------------------------------------
+def opt_table():
+    buf = (ctypes.c_int * 512)()
+    n = lib().orc_opt_table(buf, 512)
+    return [buf[i] for i in range(n)]
+
+
+def first_greater(x):
+    return lib().orc_first_greater(int(x))
+
+
+def opt_size(x):
+    return lib().orc_opt_size(int(x))
+
+
+def pair_count(wl, wh, wi):
+    return lib().orc_pair_count(float(wl), float(wh), float(wi))
+
+
+def fft1d(x, sign=-1):
+    a = _c128(x).copy()
+    lib().orc_fft1d(a.size, a.view(np.float64).ctypes.data_as(_dp), int(sign))
+    return a
+
+
+def fft2d(x, sign=-1):
+    a = _c128(x).copy()
+    r, c = a.shape
+    lib().orc_fft2d(r, c, a.view(np.float64).ctypes.data_as(_dp), int(sign))
+    return a
+
+
+def normal(seed, t, p):
+    return lib().orc_normal(seed, t, p)
+
+
+def gen_phase(W, H, amp, carrier):
+    phi = np.empty((H, W), np.float64)
+    lib().orc_gen_phase(W, H, float(amp), float(carrier), _ptr(phi))
+    return phi
+
+
+def gen_frame(W, H, t, seed, amp0=3.0, amp1=0.1, carrier=0.05, noise=10.0):
+    """Return (I, phi) float64 arrays of shape (H, W) for frame t."""
+    I = np.empty((H, W), np.float64)
+    phi = np.empty((H, W), np.float64)
+    lib().orc_gen_frame(W, H, int(t), int(seed), float(amp0), float(amp1),
+                        float(carrier), float(noise), _ptr(I), _ptr(phi))
+    return I, phi
+
+
+def gen_sequence(W, H, T, seed, amp0=3.0, amp1=0.1, carrier=0.05, noise=10.0,
+                 dtype=np.float64):
+    """Frames (T, H, W) in `dtype` plus phi(.,.,0) in float64."""
+    frames = np.empty((T, H, W), dtype)
+    phi0 = None
+    for t in range(T):
+        I, phi = gen_frame(W, H, t, seed, amp0, amp1, carrier, noise)
+        frames[t] = I.astype(dtype)
+        if t == 0:
+            phi0 = phi
+    return frames, phi0
+
+
+class WFF2:
+    """FP64 WFF2 (WFT2_cpu in WFF mode)."""
+
+    def __init__(self, W, H, sigx=10.0, wxl=-0.2, wxh=0.2, wxi=0.1,
+                 sigy=None, wyl=None, wyh=None, wyi=None, thr=-1.0,
+                 nthreads=None, separable=True):
+        sigy = sigx if sigy is None else sigy
+        wyl = wxl if wyl is None else wyl
+        wyh = wxh if wyh is None else wyh
+        wyi = wxi if wyi is None else wyi
+        nthreads = nthreads or min(os.cpu_count() or 1, 16)
+        self.W, self.H = W, H
+        self._h = lib().orc_wft2_create(W, H, sigx, wxl, wxh, wxi, sigy, wyl, wyh,
+                                        wyi, thr, nthreads, 1 if separable else 0)
+        if not self._h:
+            raise ValueError("FFT padding is out of range [4096]")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_wft2_destroy(self._h)
+            self._h = None
+
+    @property
+    def padded(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        lib().orc_wft2_padded(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    @property
+    def halfwin(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        lib().orc_wft2_halfwin(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    @property
+    def pairs(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        lib().orc_wft2_pairs(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    @property
+    def thr(self):
+        return lib().orc_wft2_thr(self._h)
+
+    def __call__(self, f):
+        f = _c128(f)
+        assert f.shape == (self.H, self.W)
+        out = np.empty_like(f)
+        lib().orc_wft2_compute(self._h, f.view(np.float64).ctypes.data_as(_dp),
+                               out.view(np.float64).ctypes.data_as(_dp))
+        return out
+
+
+class DPRA:
+    """FP64 DPRA_CPU chain (LS + WFF2 + update)."""
+
+    def __init__(self, phi0, rr=1, sigx=10.0, wxl=-0.2, wxh=0.2, wxi=0.1,
+                 sigy=None, wyl=None, wyh=None, wyi=None, thr=-1.0,
+                 nthreads=None, float_cast=True):
+        phi0 = _f64(phi0)
+        H, W = phi0.shape
+        sigy = sigx if sigy is None else sigy
+        wyl = wxl if wyl is None else wyl
+        wyh = wxh if wyh is None else wyh
+        wyi = wxi if wyi is None else wyi
+        nthreads = nthreads or min(os.cpu_count() or 1, 16)
+        self.W, self.H = W, H
+        self._h = lib().orc_dpra_create(_ptr(phi0), W, H, rr, sigx, wxl, wxh, wxi,
+                                        sigy, wyl, wyh, wyi, thr, nthreads,
+                                        0 if float_cast else 1)
+        if not self._h:
+            raise ValueError("invalid DPRA parameters or FFT padding out of range")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_dpra_destroy(self._h)
+            self._h = None
+
+    @property
+    def thr(self):
+        return lib().orc_dpra_thr(self._h)
+
+    def ls(self, frame):
+        f = _f64(frame)
+        z = np.empty(f.shape, np.complex128)
+        lib().orc_dpra_ls(self._h, _ptr(f), z.view(np.float64).ctypes.data_as(_dp))
+        return z
+
+    def dpra_per_frame(self, frame):
+        f = _f64(frame)
+        dphi = np.empty(f.shape, np.float64)
+        phic = np.empty(f.shape, np.float64)
+        lib().orc_dpra_frame(self._h, _ptr(f), _ptr(dphi), _ptr(phic))
+        return dphi, phic
+
+    def update_ref_phi(self):
+        lib().orc_dpra_update_ref(self._h)
+
+    def run(self, frames):
+        fr = _f64(frames)
+        T = fr.shape[0]
+        out = np.empty(fr.shape, np.float64)
+        lib().orc_dpra_run(self._h, _ptr(fr), T, _ptr(out))
+        return out
+
+
+def solve3_sym(A, b):
+    """A: 3x3 (upper triangle used), b: 3 -> (x, singular)."""
+    Acm = np.ascontiguousarray(np.asarray(A, np.float64).T).ravel()  # col-major
+    bb = _f64(b).copy()
+    s = lib().orc_solve3_sym(_ptr(Acm), _ptr(bb))
+    return bb, bool(s)
+
+
+def wrapped_err(a, b):
+    a = _f64(a).ravel()
+    b = _f64(b).ravel()
+    r, m = ctypes.c_double(), ctypes.c_double()
+    lib().orc_wrapped_err(_ptr(a), _ptr(b), a.size, ctypes.byref(r), ctypes.byref(m))
+    return r.value, m.value

@@ -1,21 +1,26 @@
 // fft_device.h -- wave-level complex FFTs for gfx950 (CDNA4), FP32.
 //
-// A transform of length N = T * E is owned by T lanes of one wavefront; lane
-// t keeps the E elements x[t + T*j] (j < E) in VGPRs.  The first Stockham
-// stage is an E-point DFT done entirely in registers (compile-time twiddles);
-// the remaining stages are radix-R butterflies (R | E, product of the R's = T)
-// that exchange through a per-transform LDS buffer with one padding element
-// every E elements (keeps the stride-E scatter of stage 1 free of bank
-// conflicts).  Every stage reads the natural "t + T*j" pattern, so the last
-// stage leaves X[t + T*j] in register j: spectra and signals are always held
-// in the same lane/register layout, and pointwise work between transforms
-// (spectrum multiplies, hard threshold) needs no data movement.
-//
-// Stockham indexing (radix r, L = product of earlier radices, b = butterfly):
-//   in  = x[b + m*N/r],  twiddle W_{L*r}^{(b mod L)*m},
-//   out = y[(b div L)*L*r + (b mod L) + m*L].
-// Published algorithm: T. G. Stockham, "High-speed convolution and
-// correlation" (1966) autosort formulation.
+// A transform of length N = T * E is owned by T lanes of one wavefront.
+// Four-step decomposition (n = n1 + T n2, k = k1 + E k2):
+//   X[k1 + E k2] = sum_n1 W_T^{n1 k2} W_N^{n1 k1} sum_n2 x[n1 + T n2] W_E^{n2 k1}
+// Stage A: lane n1 holds x[n1 + T n2] in register n2 ("spatial layout") and
+//   does the E-point DFT in registers (compile-time twiddles), then multiplies
+//   by W_N^{n1 k1} (per-lane constants kept in registers).
+// Stage B: E independent T-point DFTs across lanes (batch index k1), done as
+//   Stockham radix-R stages (R | E, product of the R's = T) that exchange
+//   through LDS laid out as [k1][position].  In every stage a lane owns ONE
+//   butterfly position c and E/R values of k1, so its twiddles are a few
+//   per-lane constants and every LDS address is lane_base + compile-time
+//   offset (no per-element index arithmetic to keep live).
+// The last stage leaves the spectrum in registers ("spectral layout": lane
+//   c + (T/R) g, register i + m (E/R) holds X[(g + R i) + E (c + m T/R)]).
+// The inverse runs the stages in mirror order (inverse butterflies, conjugate
+//   twiddles) from the spectral layout back to the spatial layout, so
+//   FFT -> pointwise -> IFFT never reorders data.
+// Stockham step (radix r, L = product of earlier radices, butterfly c):
+//   in = z[c + m T/r] * W_{L r}^{(c mod L) m};  out = z'[(c div L) L r + c mod L + m L].
+// Published algorithms: Cooley-Tukey four-step (Bailey 1990) and the
+// Stockham autosort formulation.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <type_traits>
@@ -220,88 +225,203 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// ---------------------------------------------------- cross-lane FFT plan
+// ---------------------------------------------------------------- plans
+template <int... R>
+struct RadixList {
+  static constexpr int n = sizeof...(R);
+  static constexpr int v[n] = {R...};
+  static constexpr int at(int s) { return v[s]; }
+  static constexpr int prod_before(int s) {
+    int p = 1;
+    for (int i = 0; i < s; ++i) p *= v[i];
+    return p;
+  }
+  static constexpr int maxr() {
+    int m = 1;
+    for (int i = 0; i < n; ++i) m = v[i] > m ? v[i] : m;
+    return m;
+  }
+};
+
 // FftPlan<T, E, R...>: N = T*E; R... are the cross-lane radices (each divides
-// E, product == T).
+// E, product == T).  LDS per transform: E rows of ROW = T + pad elements.
 template <int T_, int E_, int... R>
 struct FftPlan {
   static constexpr int T = T_;
   static constexpr int E = E_;
   static constexpr int N = T_ * E_;
-  static constexpr int LDS = N + N / E_;  // padded element count
+  using Rad = RadixList<R...>;
+  static constexpr int NS = sizeof...(R);
+  static constexpr int RL = Rad::at(NS - 1);  // last radix
+  static constexpr int ROW = T_ + 4;          // row pitch (elements) of the [k1][pos] buffer
+  static constexpr int LDS = E_ * ROW;
   static_assert(((E_ % R == 0) && ...), "every cross-lane radix must divide E");
   static_assert((R * ... * 1) == T_, "cross-lane radices must multiply to T");
-  __device__ __forceinline__ static int pad(int i) { return i + i / E_; }
+  // spectral layout: k of register j in lane (c, g) = kbase(lane) + koff(j)
+  __device__ __forceinline__ static int kbase(int lane) {
+    const int c = lane % (T_ / RL), g = lane / (T_ / RL);
+    return g + E_ * c;
+  }
+  static constexpr int koff(int j) {
+    const int nb = E_ / RL, i = j % nb, m = j / nb;
+    return RL * i + E_ * (T_ / RL) * m;
+  }
 };
 
-template <class P, int R, int L, bool LAST, bool INV>
-__device__ __forceinline__ void cross_stage(float2 (&v)[P::E], float2* buf, const float2* tw, int lane) {
-  constexpr int T = P::T, E = P::E, N = P::N;
-  constexpr int NB = E / R;          // butterflies per lane
-  constexpr int TS = N / (L * R);    // twiddle table stride
-  // gather natural-order inputs
-  sfor<0, E>([&](auto jc) {
-    constexpr int j = decltype(jc)::value;
-    v[j] = buf[P::pad(lane + T * j)];
+// Per-lane constants of one plan: transition twiddles W_N^{lane k1} and the
+// stage twiddles W_{L R}^{(c mod L) m}, read once from the W_N^k table.
+// TT_LDS: the transition twiddles live in a per-workgroup LDS table
+// ([k1][lane], shared by every transform of the workgroup) instead of E VGPR
+// pairs; kernels that hold several spectra in registers use it.
+template <class P, bool TT_LDS = false>
+struct FftLane {
+  static constexpr int T = P::T, E = P::E, N = P::N, NS = P::NS;
+  static constexpr int RM = P::Rad::maxr();
+  static constexpr bool kTTLds = TT_LDS;
+  int lane;
+  float2 tt[TT_LDS ? 1 : E];
+  const float2* ttab;  // LDS [E][T] table when TT_LDS
+  float2 ts[NS][RM];
+  // TT_LDS: ttab_lds must hold E*T entries; the whole workgroup must call
+  // fill_ttab + __syncthreads before any transform.
+  __device__ __forceinline__ static void fill_ttab(float2* ttab_lds, const float2* __restrict__ twN) {
+    for (int i = threadIdx.x; i < E * T; i += blockDim.x) {
+      const int k = i / T, l = i % T;
+      ttab_lds[i] = twN[(l * k) % N];
+    }
+  }
+  __device__ __forceinline__ float2 tw_trans(int k) const {
+    if constexpr (TT_LDS) {
+      // opaque lane offset: re-read per transform rather than cache in VGPRs
+      int o = lane;
+      __asm__ volatile("" : "+v"(o));
+      return ttab[k * T + o];
+    } else {
+      return tt[k];
+    }
+  }
+  __device__ __forceinline__ void init(int lane_, const float2* __restrict__ twN,
+                                       const float2* ttab_lds = nullptr) {
+    lane = lane_;
+    ttab = ttab_lds;
+    if constexpr (!TT_LDS) {
+#pragma unroll
+      for (int k = 0; k < E; ++k) tt[k] = twN[(lane * k) % N];
+    }
+    sfor<0, NS>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      constexpr int R = P::Rad::at(s), L = P::Rad::prod_before(s);
+      const int c = lane % (T / R);
+#pragma unroll
+      for (int m = 0; m < R; ++m) ts[s][m] = twN[((c % L) * m * (N / (L * R))) % N];
+    });
+  }
+};
+
+template <class P, int S, bool INV, class Ctx>
+__device__ __forceinline__ void stage_fwd(float2 (&v)[P::E], float2* buf, const Ctx& L_) {
+  // forward Stockham stage S: read z[k1][c + m T/R] -> twiddle -> DFT_R -> write
+  constexpr int T = P::T, E = P::E, ROW = P::ROW;
+  constexpr int R = P::Rad::at(S), L = P::Rad::prod_before(S), NB = E / R;
+  constexpr bool LAST = (S == P::NS - 1);
+  const int lane = L_.lane;
+  const int c = lane % (T / R), g = lane / (T / R);
+  const float2* rb = buf + g * ROW + c;
+  sfor<0, NB>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    sfor<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      v[i + m * NB] = rb[R * i * ROW + m * (T / R)];
+    });
   });
   wave_sync();
-  sfor<0, NB>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    const int b = lane + T * q;
-    const int jm = b % L;
+  float2* wb = buf + g * ROW + (c / L) * L * R + (c % L);
+  sfor<0, NB>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
     float2 u[R];
     sfor<0, R>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
-      float2 a = v[q + m * NB];
-      if constexpr (m > 0) {
-        float2 w = tw[jm * m * TS];
-        a = INV ? cmulc(a, w) : cmul(a, w);
-      }
+      float2 a = v[i + m * NB];
+      if constexpr (L > 1 && m > 0) a = INV ? cmulc(a, L_.ts[S][m]) : cmul(a, L_.ts[S][m]);
       u[m] = a;
     });
     Dft<R, INV>::run(u);
-    if constexpr (LAST) {
-      // out index b + m*N/R == lane + T*(q + m*NB): stays in this lane
-      sfor<0, R>([&](auto mc) { v[q + decltype(mc)::value * NB] = u[decltype(mc)::value]; });
-    } else {
-      const int base = (b - jm) * R + jm;
-      sfor<0, R>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        buf[P::pad(base + m * L)] = u[m];
-      });
-    }
+    sfor<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      if constexpr (LAST) v[i + m * NB] = u[m];
+      else wb[R * i * ROW + m * L] = u[m];
+    });
   });
   if constexpr (!LAST) wave_sync();
 }
 
-template <class P, bool INV, int L, int R0, int... Rs>
-__device__ __forceinline__ void cross_stages(float2 (&v)[P::E], float2* buf, const float2* tw, int lane) {
-  cross_stage<P, R0, L, sizeof...(Rs) == 0, INV>(v, buf, tw, lane);
-  if constexpr (sizeof...(Rs) > 0) cross_stages<P, INV, L * R0, Rs...>(v, buf, tw, lane);
+template <class P, int S, class Ctx>
+__device__ __forceinline__ void stage_inv(float2 (&v)[P::E], float2* buf, const Ctx& L_) {
+  // mirror of stage_fwd: read outputs -> inverse DFT_R -> conj twiddle -> write inputs
+  constexpr int T = P::T, E = P::E, ROW = P::ROW;
+  constexpr int R = P::Rad::at(S), L = P::Rad::prod_before(S), NB = E / R;
+  constexpr bool LAST = (S == P::NS - 1);
+  const int lane = L_.lane;
+  const int c = lane % (T / R), g = lane / (T / R);
+  if constexpr (!LAST) {
+    const float2* rb = buf + g * ROW + (c / L) * L * R + (c % L);
+    sfor<0, NB>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      sfor<0, R>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        v[i + m * NB] = rb[R * i * ROW + m * L];
+      });
+    });
+    wave_sync();
+  }
+  float2* wb = buf + g * ROW + c;
+  sfor<0, NB>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    float2 u[R];
+    sfor<0, R>([&](auto mc) { u[decltype(mc)::value] = v[i + decltype(mc)::value * NB]; });
+    Dft<R, true>::run(u);
+    sfor<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      float2 a = u[m];
+      if constexpr (L > 1 && m > 0) a = cmulc(a, L_.ts[S][m]);
+      wb[R * i * ROW + m * (T / R)] = a;
+    });
+  });
+  wave_sync();
 }
 
-template <class P>
-struct FftRun;
+template <class P, int S, class Ctx>
+__device__ __forceinline__ void stages_inv(float2 (&v)[P::E], float2* buf, const Ctx& L_) {
+  stage_inv<P, S>(v, buf, L_);
+  if constexpr (S > 0) stages_inv<P, S - 1>(v, buf, L_);
+}
 
-template <int T, int E, int... R>
-struct FftRun<FftPlan<T, E, R...>> {
-  using P = FftPlan<T, E, R...>;
-  // In:  v[j] = x[lane + T*j].  Out: v[j] = X[lane + T*j].  Unnormalised.
-  // buf: this transform's LDS region (P::LDS entries); tw: W_N^k, k < N.
-  template <bool INV>
-  __host__ __device__ __forceinline__ static void run(float2 (&v)[E], float2* buf, const float2* tw, int lane) {
-    Dft<E, INV>::run(v);
-    if constexpr (sizeof...(R) == 0) {
-      return;
-    } else {
-      sfor<0, E>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        buf[P::pad(lane * E + m)] = v[m];
-      });
-      wave_sync();
-      cross_stages<P, INV, E, R...>(v, buf, tw, lane);
-    }
-  }
-};
+// Forward DFT (unnormalised): spatial layout in, spectral layout out.
+template <class P, class Ctx>
+__device__ __forceinline__ void fft_fwd(float2 (&v)[P::E], float2* buf, const Ctx& L_) {
+  constexpr int E = P::E, ROW = P::ROW;
+  Dft<E, false>::run(v);
+  float2* wb = buf + L_.lane;
+  sfor<0, E>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    wb[k * ROW] = (k == 0) ? v[k] : cmul(v[k], L_.tw_trans(k));
+  });
+  wave_sync();
+  sfor<0, P::NS>([&](auto sc) { stage_fwd<P, decltype(sc)::value, false>(v, buf, L_); });
+}
+
+// Inverse DFT (unnormalised, e^{+i}): spectral layout in, spatial layout out.
+template <class P, class Ctx>
+__device__ __forceinline__ void fft_inv(float2 (&v)[P::E], float2* buf, const Ctx& L_) {
+  constexpr int E = P::E, ROW = P::ROW;
+  stages_inv<P, P::NS - 1>(v, buf, L_);
+  const float2* rb = buf + L_.lane;
+  sfor<0, E>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    const float2 a = rb[k * ROW];
+    v[k] = (k == 0) ? a : cmulc(a, L_.tw_trans(k));
+  });
+  Dft<E, true>::run(v);
+}
 
 }  // namespace glsu

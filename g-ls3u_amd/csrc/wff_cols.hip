@@ -5,7 +5,7 @@ namespace glsu {
 
 hipError_t launch_cols(const WffPlan& p, hipStream_t s) {
   const int waves = p.cols_waves;
-  const int smem = plan_smem_bytes(p.plany, waves);
+  const int smem = plan_smem_bytes(p.plany, waves, true);
   switch (p.plany) {
 #define BODY                                                                                    \
   {                                                                                             \

@@ -64,25 +64,28 @@ __global__ __launch_bounds__(64 * WAVES) void wff_rows_fwd_kernel(
     const float2* __restrict__ tw_g, int W, int H, int nwx, size_t plane) {
   constexpr int T = P::T, E = P::E, N = P::N;
   extern __shared__ float2 smem[];
-  float2* tw = smem;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = tw_g[i];
-  __syncthreads();
   const int slot = threadIdx.x / T, lane = threadIdx.x % T;
-  float2* buf = smem + N + slot * P::LDS;
+  float2* buf = smem + slot * P::LDS;
   const int y = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / T) + slot;
   if (y >= H) return;
+  FftLane<P> ctx;
+  ctx.init(lane, tw_g);
+  const int kb = P::kbase(lane);
   float2 z[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     const int x = lane + T * j;
     z[j] = x < W ? f[(size_t)y * W + x] : make_float2(0.f, 0.f);
   }
-  FftRun<P>::template run<false>(z, buf, tw, lane);
+  fft_fwd<P>(z, buf, ctx);
   for (int k = 0; k < nwx; ++k) {
+    const float2* hx = Hx + (size_t)k * N + kb;
     float2 v[E];
-#pragma unroll
-    for (int j = 0; j < E; ++j) v[j] = cmul(z[j], Hx[(size_t)k * N + lane + T * j]);
-    FftRun<P>::template run<true>(v, buf, tw, lane);
+    sfor<0, E>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      v[j] = cmul(z[j], hx[P::koff(j)]);
+    });
+    fft_inv<P>(v, buf, ctx);
     float2* out = A + plane * k;
 #pragma unroll
     for (int j = 0; j < E; ++j) {
@@ -103,16 +106,20 @@ __global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
     int nwy, size_t plane) {
   constexpr int T = P::T, E = P::E, N = P::N;
   extern __shared__ float2 smem[];
-  float2* tw = smem;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = tw_g[i];
+  using Ctx = FftLane<P, true>;
+  float2* ttab = smem;
+  Ctx::fill_ttab(ttab, tw_g);
   __syncthreads();
   const int slot = threadIdx.x / T, lane = threadIdx.x % T;
-  float2* buf = smem + N + slot * P::LDS;
+  float2* buf = smem + E * T + slot * P::LDS;
   const int item = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / T) + slot;
   if (item >= nwx * W) return;
   const int k = item / W, x = item % W;
   const float thr = *thr_p;
   const float thr2 = thr * thr;
+  Ctx ctx;
+  ctx.init(lane, tw_g, ttab);
+  const int kb = P::kbase(lane);
   const float2* colA = A + plane * k;
   float2 ah[E], acc[E];
 #pragma unroll
@@ -121,28 +128,35 @@ __global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
     ah[j] = y < H ? colA[tile_index(y, x, H)] : make_float2(0.f, 0.f);
     acc[j] = make_float2(0.f, 0.f);
   }
-  FftRun<P>::template run<false>(ah, buf, tw, lane);
+  fft_fwd<P>(ah, buf, ctx);
   for (int q = 0; q < nwy; ++q) {
-    const float2* hy = Hy + (size_t)q * N;
+    const float2* hy = Hy + (size_t)q * N + kb;
     float2 v[E];
-#pragma unroll
-    for (int j = 0; j < E; ++j) v[j] = cmul(ah[j], hy[lane + T * j]);
-    FftRun<P>::template run<true>(v, buf, tw, lane);
+    sfor<0, E>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      v[j] = cmul(ah[j], hy[P::koff(j)]);
+    });
+    fft_inv<P>(v, buf, ctx);
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       const int y = lane + T * j;
       const float m2 = v[j].x * v[j].x + v[j].y * v[j].y;
       if (!(y < H && m2 >= thr2)) v[j] = make_float2(0.f, 0.f);
     }
-    FftRun<P>::template run<false>(v, buf, tw, lane);
-#pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const float2 h = hy[lane + T * j];
+    fft_fwd<P>(v, buf, ctx);
+    // re-read the window spectrum instead of keeping 2E VGPRs live across the
+    // two transforms: the opaque copy of the lane offset defeats load CSE
+    int kb2 = kb;
+    __asm__ volatile("" : "+v"(kb2));
+    const float2* hy2 = Hy + (size_t)q * N + kb2;
+    sfor<0, E>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      const float2 h = hy2[P::koff(j)];
       acc[j].x += v[j].x * h.x - v[j].y * h.y;
       acc[j].y += v[j].x * h.y + v[j].y * h.x;
-    }
+    });
   }
-  FftRun<P>::template run<true>(acc, buf, tw, lane);
+  fft_inv<P>(acc, buf, ctx);
   float2* colD = D + plane * k;
 #pragma unroll
   for (int j = 0; j < E; ++j) {
@@ -162,13 +176,13 @@ __global__ __launch_bounds__(64 * WAVES) void wff_rows_inv_kernel(
     UpdateArgs u) {
   constexpr int T = P::T, E = P::E, N = P::N;
   extern __shared__ float2 smem[];
-  float2* tw = smem;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = tw_g[i];
-  __syncthreads();
   const int slot = threadIdx.x / T, lane = threadIdx.x % T;
-  float2* buf = smem + N + slot * P::LDS;
+  float2* buf = smem + slot * P::LDS;
   const int y = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / T) + slot;
   if (y >= H) return;
+  FftLane<P> ctx;
+  ctx.init(lane, tw_g);
+  const int kb = P::kbase(lane);
   float2 acc[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) acc[j] = make_float2(0.f, 0.f);
@@ -180,15 +194,16 @@ __global__ __launch_bounds__(64 * WAVES) void wff_rows_inv_kernel(
       const int x = lane + T * j;
       v[j] = x < W ? Dk[tile_index(y, x, H)] : make_float2(0.f, 0.f);
     }
-    FftRun<P>::template run<false>(v, buf, tw, lane);
-#pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const float2 h = Hx[(size_t)k * N + lane + T * j];
+    fft_fwd<P>(v, buf, ctx);
+    const float2* hx = Hx + (size_t)k * N + kb;
+    sfor<0, E>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      const float2 h = hx[P::koff(j)];
       acc[j].x += v[j].x * h.x - v[j].y * h.y;
       acc[j].y += v[j].x * h.y + v[j].y * h.x;
-    }
+    });
   }
-  FftRun<P>::template run<true>(acc, buf, tw, lane);
+  fft_inv<P>(acc, buf, ctx);
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     const int x = lane + T * j;
@@ -221,9 +236,12 @@ __global__ __launch_bounds__(64 * WAVES) void wff_rows_inv_kernel(
 }
 
 // ============================================================ host side
-static int plan_smem_bytes(int id, int waves) {
+static int plan_smem_bytes(int id, int waves, bool ttab = false) {
+  // per transform: E rows of (T + 4) elements (the FftPlan::LDS layout), plus
+  // the shared E x T transition-twiddle table when the kernel keeps it in LDS
   for (auto& p : fft_plans())
-    if (p.id == id) return (int)sizeof(float2) * (p.N + (waves * 64 / p.T) * (p.N + p.N / p.E));
+    if (p.id == id)
+      return (int)sizeof(float2) * ((waves * 64 / p.T) * p.E * (p.T + 4) + (ttab ? p.E * p.T : 0));
   return 0;
 }
 

@@ -158,7 +158,7 @@ int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh,
   p.planx = planx; p.plany = plany;
   p.Nx = px->N; p.Ny = py->N;
   // keep dynamic LDS per workgroup within 64 KiB
-  while (p.cols_waves > 1 && plan_smem_bytes(plany, p.cols_waves) > 65536) p.cols_waves /= 2;
+  while (p.cols_waves > 1 && plan_smem_bytes(plany, p.cols_waves, true) > 65536) p.cols_waves /= 2;
   while (p.rows_waves > 1 && plan_smem_bytes(planx, p.rows_waves) > 65536) p.rows_waves /= 2;
   const int wb = (W + kTileCols - 1) / kTileCols;
   p.plane_elems = (size_t)wb * kTileCols * H;

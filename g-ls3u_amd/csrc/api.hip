@@ -141,19 +141,32 @@ static int default_plan(int need) {
   return (firstN < 0.8 * bestN64) ? first : best64;
 }
 
-static int plan_common(WffPlan& p, int W, int H, const glsu_wff_params* pr, int planx, int plany) {
+// Host-only validation (no device needed): sizes, parameters, padding rule.
+static int validate(int W, int H, const glsu_wff_params* pr, int* planx, int* plany) {
   if (W < 1 || H < 1) return fail(GLSU_ERR_ARG, "invalid size %dx%d", W, H);
-  if (!(pr->wxi > 0) || !(pr->wyi > 0) || !(pr->sigma_x > 0) || !(pr->sigma_y > 0))
+  if (!(pr->wxi > 0) || !(pr->wyi > 0) || !(pr->sigma_x > 0) || !(pr->sigma_y > 0) ||
+      !(pr->wxh >= pr->wxl) || !(pr->wyh >= pr->wyl))
     return fail(GLSU_ERR_ARG, "invalid WFF parameters");
   const int sx = (int)std::round(3 * pr->sigma_x), sy = (int)std::round(3 * pr->sigma_y);
   if (!reference_padding_ok(W, H, sx, sy))
     return fail(GLSU_ERR_PADDING,
                 "FFT padding is out of range [4096]. Shrink the size of either the image or the "
                 "Gaussian Window!");
-  if (planx < 0) planx = default_plan(W + sx);
-  if (plany < 0) plany = default_plan(H + sy);
-  if (planx < 0 || plany < 0)
+  if (*planx < 0) *planx = default_plan(W + sx);
+  if (*plany < 0) *plany = default_plan(H + sy);
+  if (*planx < 0 || *plany < 0)
     return fail(GLSU_ERR_PADDING, "no compiled FFT length covers %dx%d + half window", W, H);
+  bool okx = false, oky = false;
+  for (auto& q : fft_plans()) {
+    if (q.id == *planx && q.N >= W + sx) okx = true;
+    if (q.id == *plany && q.N >= H + sy) oky = true;
+  }
+  if (!okx || !oky) return fail(GLSU_ERR_ARG, "FFT plan too short for image + half window");
+  return GLSU_OK;
+}
+
+static int plan_common(WffPlan& p, int W, int H, const glsu_wff_params* pr, int planx, int plany) {
+  if (int rc = validate(W, H, pr, &planx, &plany)) return rc;
   std::string err;
   int rc = wff_plan_init(p, W, H, pr->sigma_x, pr->wxl, pr->wxh, pr->wxi, pr->sigma_y, pr->wyl,
                          pr->wyh, pr->wyi, pr->thr, planx, plany, err);
@@ -179,6 +192,10 @@ extern "C" int glsu_wft2_create_plan(glsu_wft2** out, int W, int H, int type,
   *out = nullptr;
   if (type == GLSU_WFR) return fail(GLSU_ERR_UNSUPPORTED, "WFR2 mode is not provided");
   if (type != GLSU_WFF) return fail(GLSU_ERR_ARG, "unknown WFT type %d", type);
+  {
+    int px = planx, py = plany;
+    if (int rc = validate(W, H, pr, &px, &py)) return rc;
+  }
   if (int rc = check_device()) return rc;
   auto* h = new glsu_wft2();
   int rc = plan_common(h->p, W, H, pr, planx, plany);
@@ -359,12 +376,16 @@ static int dpra_create_impl(glsu_dpra** out, const PT* phi0, int W, int H, int r
   if (!out || !phi0) return fail(GLSU_ERR_ARG, "null argument");
   *out = nullptr;
   if (rr < 1) return fail(GLSU_ERR_ARG, "reference update rate must be >= 1");
-  if (int rc = check_device()) return rc;
   glsu_wff_params def;
   if (!pr) {
     glsu_wff_params_dpra_reference(&def);
     pr = &def;
   }
+  {
+    int px = planx, py = plany;
+    if (int rc = validate(W, H, pr, &px, &py)) return rc;
+  }
+  if (int rc = check_device()) return rc;
   auto* h = new glsu_dpra();
   h->W = W; h->H = H; h->rr = rr;
   int rc = plan_common(h->p, W, H, pr, planx, plany);

@@ -1,0 +1,96 @@
+"""C-ABI checks that need no GPU: libglsu.so loads, exports every entry point
+include/glsu.h declares, and reports argument / padding errors the way the
+reference constructors do (status codes instead of `throw -1`)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import glsu
+from glsu import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "glsu.h")
+
+
+def declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(glsu_\w+)\s*\(", txt)))
+
+
+def test_header_declarations_match_binding():
+    assert declared() == sorted(_lib.exported_symbols())
+
+
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (glsu_\w+)", out))
+    missing = [s for s in declared() if s not in exported]
+    assert not missing, missing
+    L = glsu.lib()
+    for s in declared():
+        assert hasattr(L, s)
+
+
+def test_version_and_plans():
+    assert b"gfx950" in glsu.lib().glsu_version()
+    plans = glsu.fft_plans()
+    assert all(p["N"] == p["T"] * p["E"] for p in plans)
+    assert any(p["N"] >= 1024 + 30 for p in plans)
+    assert any(p["N"] >= 2048 + 36 for p in plans)
+
+
+def test_param_helpers():
+    p = glsu.wft2_default_params()
+    assert (p.sigma_x, p.wxl, p.wxh, p.wxi, p.thr) == (10.0, -2.3, 2.3, 0.1, -1.0)
+    q = glsu.dpra_reference_params()
+    assert q.sigma_x == 20 and q.thr == 15 and q.wxl == float(ctypes.c_float(-0.15).value)
+
+
+def _create_wft2(W, H, typ=0, params=None):
+    h = ctypes.c_void_p()
+    p = params or glsu.wff_params()
+    return glsu.lib().glsu_wft2_create(ctypes.byref(h), W, H, typ, ctypes.byref(p), None), h
+
+
+def test_padding_error_code():
+    # W + 2s > 4096: the reference throws int -1 (WFT/WFT2_CPU.cpp:69-73)
+    rc, h = _create_wft2(4096 - 60 + 1, 32)
+    assert rc == _lib.ERR_PADDING
+    assert b"4096" in glsu.lib().glsu_last_error()
+
+
+def test_argument_errors():
+    assert _create_wft2(0, 10)[0] == _lib.ERR_ARG
+    assert _create_wft2(10, 10, typ=1)[0] == _lib.ERR_UNSUPPORTED      # WFR2 not provided
+    assert _create_wft2(10, 10, typ=7)[0] == _lib.ERR_ARG
+    assert _create_wft2(10, 10, params=glsu.wff_params(wxi=0.0))[0] == _lib.ERR_ARG
+    h = ctypes.c_void_p()
+    phi = (ctypes.c_double * 4)()
+    rc = glsu.lib().glsu_dpra_create(ctypes.byref(h), phi, 2, 2, 0, None, None)
+    assert rc == _lib.ERR_ARG                                            # rr must be >= 1
+    assert glsu.lib().glsu_dpra_frame(None, None, 1, None, None, None) == _lib.ERR_ARG
+
+
+def test_no_device_is_reported_not_faked():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    n = ctypes.c_int(-1)
+    glsu.lib().glsu_device_count(ctypes.byref(n))
+    assert n.value == 0
+    rc, _ = _create_wft2(64, 64)
+    assert rc == _lib.ERR_DEVICE
+
+
+def test_cpp_wrapper_compiles():
+    # include/glsu.hpp (reference-style C++ classes) compiles against the ABI
+    src = os.path.join(ROOT, "g-ls3u_amd", "csrc", "example_dpra.cpp")
+    out = "/tmp/glsu_example_syntax.o"
+    r = subprocess.run(["/opt/rocm/bin/hipcc", "-std=c++17", "-fsyntax-only", src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]

@@ -116,3 +116,23 @@ def test_graph_equals_eager(oracle):
         b = g(fr, graph=True).cpu()
         torch.cuda.synchronize()
         assert torch.equal(a, b), f"call {rep}: max diff {(a - b).abs().max().item()}"
+
+
+def test_every_fft_plan_matches_oracle(oracle):
+    """Each compiled FFT plan (length N = T*E), forced on both axes, gives the
+    same WFF2 result as the FP64 oracle on an image it covers."""
+    plans = glsu.fft_plans()
+    sig, lo, hi, st = 2.0, -0.6, 0.6, 0.3
+    for p in plans:
+        W = min(p["N"] - 6, 300)
+        H = max(min(p["N"] - 6, 120), 16)
+        say(f"[plan {p['id']}] T={p['T']} E={p['E']} N={p['N']} image {W}x{H}")
+        rng = np.random.default_rng(p["id"])
+        ph = 0.6 * np.arange(W)[None, :] + 0.25 * np.arange(H)[:, None] + rng.normal(0, 0.3, (H, W))
+        f = np.exp(1j * ph).astype(np.complex64)
+        ref = oracle.WFF2(W, H, sig, lo, hi, st)(f.astype(np.complex128))
+        w = glsu.WFT2(W, H, params=glsu.wff_params(sig, lo, hi, st), planx=p["id"], plany=p["id"])
+        out = w(torch.from_numpy(f).cuda()).cpu().numpy()
+        rel = np.abs(out - ref).max() / np.abs(ref).max()
+        say(f"   rel={rel:.2e}")
+        assert rel < 1e-4, p

@@ -126,8 +126,17 @@ static bool reference_padding_ok(int W, int H, int sx, int sy) {
   return W + 2 * sx <= 4096 && H + 2 * sy <= 4096;
 }
 
-static int default_plan(int need) {
+// Row transforms (x axis) use one wavefront per transform (T == 64 plans);
+// column transforms may pack several narrower transforms per wavefront.
+static int default_plan(int need, bool t64_only) {
   auto c = fft_candidates(need);
+  if (t64_only) {
+    std::vector<int> k;
+    for (int id : c)
+      for (auto& p : fft_plans())
+        if (p.id == id && p.T == 64) k.push_back(id);
+    c = k;
+  }
   if (c.empty()) return -1;
   // prefer one transform per wavefront unless a narrower plan is much shorter
   int best64 = -1, bestN64 = 0, first = c[0], firstN = 0;
@@ -152,16 +161,18 @@ static int validate(int W, int H, const glsu_wff_params* pr, int* planx, int* pl
     return fail(GLSU_ERR_PADDING,
                 "FFT padding is out of range [4096]. Shrink the size of either the image or the "
                 "Gaussian Window!");
-  if (*planx < 0) *planx = default_plan(W + sx);
-  if (*plany < 0) *plany = default_plan(H + sy);
+  if (*planx < 0) *planx = default_plan(W + sx, true);
+  if (*plany < 0) *plany = default_plan(H + sy, false);
   if (*planx < 0 || *plany < 0)
     return fail(GLSU_ERR_PADDING, "no compiled FFT length covers %dx%d + half window", W, H);
   bool okx = false, oky = false;
   for (auto& q : fft_plans()) {
-    if (q.id == *planx && q.N >= W + sx) okx = true;
+    if (q.id == *planx && q.N >= W + sx && q.T == 64) okx = true;
     if (q.id == *plany && q.N >= H + sy) oky = true;
   }
-  if (!okx || !oky) return fail(GLSU_ERR_ARG, "FFT plan too short for image + half window");
+  if (!okx || !oky)
+    return fail(GLSU_ERR_ARG,
+                "FFT plan too short for image + half window (or a narrow plan on the x axis)");
   return GLSU_OK;
 }
 

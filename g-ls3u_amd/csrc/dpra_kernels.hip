@@ -13,13 +13,16 @@
 // A = R^T M A'' M^T R with A'' = sum_q w_q w_q^T + (9 - n) w0 w0^T, w0 = [1,-1,0],
 // and b'' = sum_q I_q w_q, so A'' y = b'' gives x'_1 = y1, x'_2 = y2 and
 //   Delta = atan2(-y2, y1) - phi_p,  i.e.  z = (y1 - j y2)/|y| * e^{-j phi_p}.
-// The entries of A'' are O(d^2..d^4) quantities formed without cancellation;
-// the 9 sums are accumulated in FP32 with a compensated two-sum, the DC
-// unknown is eliminated (2x2 Schur complement) and the 2x2 system is solved by
-// Cramer's rule.  det(A) = 9 det2.  Where |det A| < 1e-6 * A00*A11*A22 the 2x2
-// solve is redone in FP64 from the compensated sums; where
-// det A <= 1e-10 * A00*A11*A22 the pixel is singular and z = 1 (the oracle's
-// defined behaviour for the case where LAPACKE_dsysv reports failure).
+// The half-angle sines come from one sincos(phi/2) per pixel and the
+// difference formula sin(d/2) = sin(a/2)cos(b/2) - cos(a/2)sin(b/2), which has
+// no cancellation for small d, so e and s' keep full relative precision.  The
+// three sums whose terms change sign (s', e s', I s') are accumulated with a
+// compensated two-sum; the other five have terms of one sign and are plain
+// FP32 sums.  The DC unknown is eliminated (2x2 Schur complement) and the 2x2
+// system is solved by Cramer's rule; det(A) = 9 det2.  Where
+// |det A| < 1e-6 * A00*A11*A22 the 2x2 solve is redone in FP64 from the
+// compensated sums; where det A <= 1e-10 * A00*A11*A22 the pixel is singular
+// and z = 1 (the oracle's defined behaviour for that case).
 #include "glsu_internal.h"
 
 #include <cstdint>
@@ -44,6 +47,27 @@ __device__ __forceinline__ float load_in(const InT* p, size_t i) {
   return (float)p[i];
 }
 
+// sin and cos of x for |x| <= pi/2 (half of a wrapped phase): Taylor series
+// to x^13 / x^14 in Horner form; truncation error < 7e-10, below FP32 rounding.
+__device__ __forceinline__ void sincos_half(float x, float* s, float* c) {
+  const float x2 = x * x;
+  float ps = -1.f / 6227020800.f;              // -1/13!
+  ps = fmaf(ps, x2, 1.f / 39916800.f);
+  ps = fmaf(ps, x2, -1.f / 362880.f);
+  ps = fmaf(ps, x2, 1.f / 5040.f);
+  ps = fmaf(ps, x2, -1.f / 120.f);
+  ps = fmaf(ps, x2, 1.f / 6.f);
+  *s = fmaf(-x * x2, ps, x);                   // x - x^3 (1/3! - x^2/5! + ...)
+  float pc = 1.f / 87178291200.f;              // 1/14!
+  pc = fmaf(pc, x2, -1.f / 479001600.f);
+  pc = fmaf(pc, x2, 1.f / 3628800.f);
+  pc = fmaf(pc, x2, -1.f / 40320.f);
+  pc = fmaf(pc, x2, 1.f / 720.f);
+  pc = fmaf(pc, x2, -1.f / 24.f);
+  pc = fmaf(pc, x2, 0.5f);
+  *c = fmaf(-x2, pc, 1.f);                     // 1 - x^2 (1/2! - x^2/4! + ...)
+}
+
 constexpr int DTX = 32, DTY = 8;
 
 template <typename InT>
@@ -52,59 +76,70 @@ __global__ __launch_bounds__(DTX* DTY) void dpra_ls_kernel(const InT* __restrict
                                                            float2* __restrict__ z, int W, int H,
                                                            int pitch) {
   __shared__ float sI[DTY + 2][DTX + 2];
-  __shared__ float sP[DTY + 2][DTX + 2];
+  __shared__ float sS[DTY + 2][DTX + 2];  // sin(phi/2)
+  __shared__ float sC[DTY + 2][DTX + 2];  // cos(phi/2)
   const int tx = threadIdx.x % DTX, ty = threadIdx.x / DTX;
   const int x0 = blockIdx.x * DTX - 1, y0 = blockIdx.y * DTY - 1;
   for (int i = threadIdx.x; i < (DTX + 2) * (DTY + 2); i += DTX * DTY) {
     const int ly = i / (DTX + 2), lx = i % (DTX + 2);
     const int gx = x0 + lx, gy = y0 + ly;
-    float vi = 0.f, vp = 0.f;
+    float vi = 0.f, hs = 0.f, hc = 1.f;
     if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
       vi = load_in(I, (size_t)gy * pitch + gx);
-      vp = phr[(size_t)gy * W + gx];
+      sincos_half(0.5f * phr[(size_t)gy * W + gx], &hs, &hc);
     }
     sI[ly][lx] = vi;
-    sP[ly][lx] = vp;
+    sS[ly][lx] = hs;
+    sC[ly][lx] = hc;
   }
   __syncthreads();
   const int x = blockIdx.x * DTX + tx, y = blockIdx.y * DTY + ty;
   if (x >= W || y >= H) return;
 
-  const float pp = sP[ty + 1][tx + 1];
-  Comp Se, Ss, See, Ses, Sss, SI, SIe, SIs;
-  Se.init(); Ss.init(); See.init(); Ses.init(); Sss.init(); SI.init(); SIe.init(); SIs.init();
+  const float hsp = sS[ty + 1][tx + 1], hcp = sC[ty + 1][tx + 1];
+  float Se = 0.f, See = 0.f, Sss = 0.f, SI = 0.f, SIe = 0.f;
+  Comp Ss, Ses, SIs;
+  Ss.init(); Ses.init(); SIs.init();
   int n = 0;
+  auto neighbour = [&](int dy, int dx) {
+    const float hsq = sS[ty + 1 + dy][tx + 1 + dx], hcq = sC[ty + 1 + dy][tx + 1 + dx];
+    // sin/cos of (phi_q - phi_p)/2 by the difference formulas
+    const float sh = fmaf(hsq, hcp, -hcq * hsp);
+    const float ch = fmaf(hcq, hcp, hsq * hsp);
+    const float sq = 2.f * sh * ch;
+    const float e = -2.f * sh * sh;
+    const float Iq = sI[ty + 1 + dy][tx + 1 + dx];
+    Se += e; See += e * e; Sss += sq * sq; SI += Iq; SIe += Iq * e;
+    Ss.add(sq); Ses.add(e * sq); SIs.add(Iq * sq);
+  };
+  if (x > 0 && x < W - 1 && y > 0 && y < H - 1) {  // interior: full neighbourhood
+    n = 9;
 #pragma unroll
-  for (int dy = -1; dy <= 1; ++dy) {
+    for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-    for (int dx = -1; dx <= 1; ++dx) {
-      const int gx = x + dx, gy = y + dy;
-      if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
-      ++n;
-      const float d = sP[ty + 1 + dy][tx + 1 + dx] - pp;
-      float sh, ch;
-      sincosf(0.5f * d, &sh, &ch);
-      const float sq = 2.f * sh * ch;
-      const float e = -2.f * sh * sh;
-      const float Iq = sI[ty + 1 + dy][tx + 1 + dx];
-      Se.add(e); Ss.add(sq);
-      See.add(e * e); Ses.add(e * sq); Sss.add(sq * sq);
-      SI.add(Iq); SIe.add(Iq * e); SIs.add(Iq * sq);
-    }
+      for (int dx = -1; dx <= 1; ++dx) neighbour(dy, dx);
+  } else {  // image border: clipped neighbourhood (4 or 6 pixels)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int gx = x + dx, gy = y + dy;
+        if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
+        ++n;
+        neighbour(dy, dx);
+      }
   }
-  float sp, cp;
-  sincosf(pp, &sp, &cp);
+  // cos/sin of phi_p by the double-angle formulas
+  const float cp = hcp * hcp - hsp * hsp, sp = 2.f * hsp * hcp;
   const float q = (float)(9 - n);
   // diagonal of the unrotated A: A11 = sum c^2, A22 = sum s^2
-  const float s1e2 = (float)n + 2.f * Se.f() + See.f();   // sum (1+e)^2
-  const float s1es = Ss.f() + Ses.f();                     // sum (1+e) s'
-  const float A11 = cp * cp * s1e2 - 2.f * cp * sp * s1es + sp * sp * Sss.f();
-  const float A22 = sp * sp * s1e2 + 2.f * cp * sp * s1es + cp * cp * Sss.f();
+  const float s1e2 = (float)n + 2.f * Se + See;   // sum (1+e)^2
+  const float s1es = Ss.f() + Ses.f();             // sum (1+e) s'
+  const float A11 = cp * cp * s1e2 - 2.f * cp * sp * s1es + sp * sp * Sss;
+  const float A22 = sp * sp * s1e2 + 2.f * cp * sp * s1es + cp * cp * Sss;
   const float dg = 9.f * A11 * A22;
 
   // FP32 Schur complement of A'' on the DC unknown
-  const float a01 = Se.f() - q, a02 = Ss.f(), a11 = See.f() + q, a12 = Ses.f(), a22 = Sss.f();
-  const float b0 = SI.f(), b1 = SIe.f(), b2 = SIs.f();
+  const float a01 = Se - q, a02 = Ss.f(), a11 = See + q, a12 = Ses.f(), a22 = Sss;
+  const float b0 = SI, b1 = SIe, b2 = SIs.f();
   const float s11 = a11 - a01 * a01 * (1.f / 9.f);
   const float s12 = a12 - a01 * a02 * (1.f / 9.f);
   const float s22 = a22 - a02 * a02 * (1.f / 9.f);
@@ -117,9 +152,9 @@ __global__ __launch_bounds__(DTX* DTY) void dpra_ls_kernel(const InT* __restrict
   if (!(fabsf(9.f * det2) >= 1e-6f * fabsf(dg))) {
     // near-singular: FP64 solve from the compensated sums
     const double qd = (double)(9 - n);
-    const double A01 = Se.d() - qd, A02 = Ss.d(), A11d = See.d() + qd, A12 = Ses.d(),
-                 A22d = Sss.d();
-    const double B0 = SI.d(), B1 = SIe.d(), B2 = SIs.d();
+    const double A01 = (double)Se - qd, A02 = Ss.d(), A11d = (double)See + qd, A12 = Ses.d(),
+                 A22d = (double)Sss;
+    const double B0 = SI, B1 = SIe, B2 = SIs.d();
     const double S11 = A11d - A01 * A01 / 9.0, S12 = A12 - A01 * A02 / 9.0,
                  S22 = A22d - A02 * A02 / 9.0;
     const double R1 = B1 - A01 * B0 / 9.0, R2 = B2 - A02 * B0 / 9.0;

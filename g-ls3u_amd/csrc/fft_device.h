@@ -241,10 +241,26 @@ struct RadixList {
     for (int i = 0; i < n; ++i) m = v[i] > m ? v[i] : m;
     return m;
   }
+  static constexpr bool uniform_pow2() {
+    for (int i = 0; i < n; ++i)
+      if (v[i] != v[0]) return false;
+    return (v[0] & (v[0] - 1)) == 0;
+  }
 };
 
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+
 // FftPlan<T, E, R...>: N = T*E; R... are the cross-lane radices (each divides
-// E, product == T).  LDS per transform: E rows of ROW = T + pad elements.
+// E, product == T).  LDS per transform: E rows of ROW elements.
+//
+// Bank-conflict-free layout for uniform power-of-two radices (T = R^NS): a
+// position p with base-R digits (d_{NS-1} ... d_0) is stored at
+//   R^{NS-1} d_{NS-1} + sum_{j < NS-1} ((d_j + d_{NS-1}) mod R) R^j,
+// and rows are ROW = T + T/R apart.  Reads (p = c + m T/R), the scattered
+// Stockham writes (p = (c div L) L R + c mod L + m L) and the transition
+// accesses then touch distinct banks across every 16- or 32-lane group
+// (checked exhaustively for the compiled plans); without the skew the stage
+// writes at stride R conflict 4-ways.
 template <int T_, int E_, int... R>
 struct FftPlan {
   static constexpr int T = T_;
@@ -253,8 +269,21 @@ struct FftPlan {
   using Rad = RadixList<R...>;
   static constexpr int NS = sizeof...(R);
   static constexpr int RL = Rad::at(NS - 1);  // last radix
-  static constexpr int ROW = T_ + 4;          // row pitch (elements) of the [k1][pos] buffer
+  static constexpr bool SKEW = Rad::uniform_pow2();
+  static constexpr int ROW = SKEW ? T_ + T_ / Rad::at(0) : T_ + 4;  // row pitch (elements)
   static constexpr int LDS = E_ * ROW;
+  __device__ __forceinline__ static int phys(int p) {
+    if constexpr (!SKEW || NS == 1) {
+      return p;
+    } else {
+      constexpr int R0 = Rad::at(0), LB = ilog2(R0), TS = LB * (NS - 1);
+      const int top = p >> TS;
+      int out = top << TS;
+#pragma unroll
+      for (int j = 0; j < NS - 1; ++j) out |= (((p >> (LB * j)) + top) & (R0 - 1)) << (LB * j);
+      return out;
+    }
+  }
   static_assert(((E_ % R == 0) && ...), "every cross-lane radix must divide E");
   static_assert((R * ... * 1) == T_, "cross-lane radices must multiply to T");
   // spectral layout: k of register j in lane (c, g) = kbase(lane) + koff(j)
@@ -326,16 +355,18 @@ __device__ __forceinline__ void stage_fwd(float2 (&v)[P::E], float2* buf, const 
   constexpr bool LAST = (S == P::NS - 1);
   const int lane = L_.lane;
   const int c = lane % (T / R), g = lane / (T / R);
-  const float2* rb = buf + g * ROW + c;
-  sfor<0, NB>([&](auto ic) {
-    constexpr int i = decltype(ic)::value;
-    sfor<0, R>([&](auto mc) {
-      constexpr int m = decltype(mc)::value;
-      v[i + m * NB] = rb[R * i * ROW + m * (T / R)];
+  const float2* rb = buf + g * ROW;
+  sfor<0, R>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    const int o = P::phys(c + m * (T / R));
+    sfor<0, NB>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      v[i + m * NB] = rb[R * i * ROW + o];
     });
   });
   wave_sync();
-  float2* wb = buf + g * ROW + (c / L) * L * R + (c % L);
+  float2* wb = buf + g * ROW;
+  const int wbase = (c / L) * L * R + (c % L);
   sfor<0, NB>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
     float2 u[R];
@@ -349,7 +380,7 @@ __device__ __forceinline__ void stage_fwd(float2 (&v)[P::E], float2* buf, const 
     sfor<0, R>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
       if constexpr (LAST) v[i + m * NB] = u[m];
-      else wb[R * i * ROW + m * L] = u[m];
+      else wb[R * i * ROW + P::phys(wbase + m * L)] = u[m];
     });
   });
   if constexpr (!LAST) wave_sync();
@@ -364,17 +395,19 @@ __device__ __forceinline__ void stage_inv(float2 (&v)[P::E], float2* buf, const 
   const int lane = L_.lane;
   const int c = lane % (T / R), g = lane / (T / R);
   if constexpr (!LAST) {
-    const float2* rb = buf + g * ROW + (c / L) * L * R + (c % L);
-    sfor<0, NB>([&](auto ic) {
-      constexpr int i = decltype(ic)::value;
-      sfor<0, R>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        v[i + m * NB] = rb[R * i * ROW + m * L];
+    const float2* rb = buf + g * ROW;
+    const int rbase = (c / L) * L * R + (c % L);
+    sfor<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      const int o = P::phys(rbase + m * L);
+      sfor<0, NB>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        v[i + m * NB] = rb[R * i * ROW + o];
       });
     });
     wave_sync();
   }
-  float2* wb = buf + g * ROW + c;
+  float2* wb = buf + g * ROW;
   sfor<0, NB>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
     float2 u[R];
@@ -384,7 +417,7 @@ __device__ __forceinline__ void stage_inv(float2 (&v)[P::E], float2* buf, const 
       constexpr int m = decltype(mc)::value;
       float2 a = u[m];
       if constexpr (L > 1 && m > 0) a = cmulc(a, L_.ts[S][m]);
-      wb[R * i * ROW + m * (T / R)] = a;
+      wb[R * i * ROW + P::phys(c + m * (T / R))] = a;
     });
   });
   wave_sync();
@@ -401,7 +434,7 @@ template <class P, class Ctx>
 __device__ __forceinline__ void fft_fwd(float2 (&v)[P::E], float2* buf, const Ctx& L_) {
   constexpr int E = P::E, ROW = P::ROW;
   Dft<E, false>::run(v);
-  float2* wb = buf + L_.lane;
+  float2* wb = buf + P::phys(L_.lane);
   sfor<0, E>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     wb[k * ROW] = (k == 0) ? v[k] : cmul(v[k], L_.tw_trans(k));
@@ -415,7 +448,7 @@ template <class P, class Ctx>
 __device__ __forceinline__ void fft_inv(float2 (&v)[P::E], float2* buf, const Ctx& L_) {
   constexpr int E = P::E, ROW = P::ROW;
   stages_inv<P, P::NS - 1>(v, buf, L_);
-  const float2* rb = buf + L_.lane;
+  const float2* rb = buf + P::phys(L_.lane);
   sfor<0, E>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     const float2 a = rb[k * ROW];

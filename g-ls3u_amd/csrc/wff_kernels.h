@@ -29,7 +29,7 @@ namespace glsu {
   X(0, 64, 4, 4, 4, 4)                   \
   X(1, 64, 8, 8, 8)                      \
   X(2, 64, 12, 4, 4, 4)                  \
-  X(3, 64, 16, 16, 4)                    \
+  X(3, 64, 16, 4, 4, 4)                  \
   X(4, 64, 20, 4, 4, 4)                  \
   X(5, 64, 24, 8, 8)                     \
   X(6, 64, 28, 4, 4, 4)                  \
@@ -40,8 +40,7 @@ namespace glsu {
   X(11, 16, 12, 4, 4)                    \
   X(12, 16, 20, 4, 4)                    \
   X(13, 32, 12, 4, 4, 2)                 \
-  X(14, 32, 20, 4, 4, 2)                 \
-  X(15, 32, 36, 4, 4, 2)
+  X(14, 32, 20, 4, 4, 2)
 
 // -------------------------------------------------------- XCD-aware order
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
@@ -52,33 +51,50 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
   return k * q + (k < r ? k : r) + local;
 }
 
+// row kernels: one wavefront per transform, up to this many waves per row
+constexpr int kRowWavesMax = 4;
+
 __device__ __forceinline__ size_t tile_index(int y, int x, int H) {
   return ((size_t)(x / kTileCols) * H + y) * kTileCols + (x % kTileCols);
 }
 
 // ---------------------------------------------------------- row forward
-// One transform per row: Zr = FFT_x(f row); for each wx: A_wx row = IFFT_x(Zr * Hx_wx).
-template <class P, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void wff_rows_fwd_kernel(
+// One workgroup per row, RW waves (one transform slot each, T == 64).
+// Wave 0 computes Zr = FFT_x(f row) and shares it through LDS; wave w then
+// produces A_wx = IFFT_x(Zr * Hx_wx) for wx = w, w + RW, ...
+template <class P>
+__global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_kernel(
     const float2* __restrict__ f, float2* __restrict__ A, const float2* __restrict__ Hx,
     const float2* __restrict__ tw_g, int W, int H, int nwx, size_t plane) {
+  static_assert(P::T == 64, "row kernels use one wavefront per transform");
   constexpr int T = P::T, E = P::E, N = P::N;
   extern __shared__ float2 smem[];
-  const int slot = threadIdx.x / T, lane = threadIdx.x % T;
-  float2* buf = smem + slot * P::LDS;
-  const int y = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / T) + slot;
+  const int RW = blockDim.x / T;
+  const int wave = threadIdx.x / T, lane = threadIdx.x % T;
+  float2* buf = smem + wave * P::LDS;
+  const int y = xcd_remap(blockIdx.x, gridDim.x);  // uniform per workgroup
   if (y >= H) return;
   FftLane<P> ctx;
   ctx.init(lane, tw_g);
   const int kb = P::kbase(lane);
   float2 z[E];
+  if (wave == 0) {
 #pragma unroll
-  for (int j = 0; j < E; ++j) {
-    const int x = lane + T * j;
-    z[j] = x < W ? f[(size_t)y * W + x] : make_float2(0.f, 0.f);
+    for (int j = 0; j < E; ++j) {
+      const int x = lane + T * j;
+      z[j] = x < W ? f[(size_t)y * W + x] : make_float2(0.f, 0.f);
+    }
+    fft_fwd<P>(z, buf, ctx);
+#pragma unroll
+    for (int j = 0; j < E; ++j) buf[j * T + lane] = z[j];  // share the spectrum
   }
-  fft_fwd<P>(z, buf, ctx);
-  for (int k = 0; k < nwx; ++k) {
+  __syncthreads();
+  if (wave != 0) {
+#pragma unroll
+    for (int j = 0; j < E; ++j) z[j] = smem[j * T + lane];
+  }
+  __syncthreads();  // wave 0 reuses its buffer below
+  for (int k = wave; k < nwx; k += RW) {
     const float2* hx = Hx + (size_t)k * N + kb;
     float2 v[E];
     sfor<0, E>([&](auto jc) {
@@ -166,19 +182,23 @@ __global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
 }
 
 // ---------------------------------------------------------- row inverse
-// acc = sum_wx FFT_x(D_wx row) * Hx_wx ; F = scale * IFFT_x(acc)
+// One workgroup per row, RW waves.  Wave w accumulates FFT_x(D_wx row) * Hx_wx
+// over wx = w, w + RW, ...; the partial spectra are summed through LDS and
+// wave 0 computes F = scale * IFFT_x(sum).
 // MODE 0: write F.  MODE 1: DPRA phase update (DPRA/dpra_cpu.cpp:244-251):
 //   t = angle(F); phicurr = wrap(phiref + t); dphi = dphiref + t (two-float sum).
-template <class P, int WAVES, int MODE>
-__global__ __launch_bounds__(64 * WAVES) void wff_rows_inv_kernel(
+template <class P, int MODE>
+__global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_inv_kernel(
     const float2* __restrict__ D, const float2* __restrict__ Hx, const float2* __restrict__ tw_g,
     int W, int H, int nwx, size_t plane, float scale, float2* __restrict__ filtered,
     UpdateArgs u) {
+  static_assert(P::T == 64, "row kernels use one wavefront per transform");
   constexpr int T = P::T, E = P::E, N = P::N;
   extern __shared__ float2 smem[];
-  const int slot = threadIdx.x / T, lane = threadIdx.x % T;
-  float2* buf = smem + slot * P::LDS;
-  const int y = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / T) + slot;
+  const int RW = blockDim.x / T;
+  const int wave = threadIdx.x / T, lane = threadIdx.x % T;
+  float2* buf = smem + wave * P::LDS;
+  const int y = xcd_remap(blockIdx.x, gridDim.x);
   if (y >= H) return;
   FftLane<P> ctx;
   ctx.init(lane, tw_g);
@@ -186,7 +206,7 @@ __global__ __launch_bounds__(64 * WAVES) void wff_rows_inv_kernel(
   float2 acc[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) acc[j] = make_float2(0.f, 0.f);
-  for (int k = 0; k < nwx; ++k) {
+  for (int k = wave; k < nwx; k += RW) {
     const float2* Dk = D + plane * k;
     float2 v[E];
 #pragma unroll
@@ -202,6 +222,23 @@ __global__ __launch_bounds__(64 * WAVES) void wff_rows_inv_kernel(
       acc[j].x += v[j].x * h.x - v[j].y * h.y;
       acc[j].y += v[j].x * h.y + v[j].y * h.x;
     });
+  }
+  if (RW > 1) {
+    if (wave != 0) {
+#pragma unroll
+      for (int j = 0; j < E; ++j) buf[j * T + lane] = acc[j];
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    for (int w = 1; w < RW; ++w) {
+      const float2* o = smem + w * P::LDS;
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const float2 a = o[j * T + lane];
+        acc[j].x += a.x;
+        acc[j].y += a.y;
+      }
+    }
   }
   fft_inv<P>(acc, buf, ctx);
 #pragma unroll
@@ -236,12 +273,29 @@ __global__ __launch_bounds__(64 * WAVES) void wff_rows_inv_kernel(
 }
 
 // ============================================================ host side
+
+template <class P>
+static constexpr int plan_lds_elems() { return P::LDS; }
+
+// FftPlan::LDS of every compiled plan, by id
+static int plan_lds(int id) {
+  switch (id) {
+#define GLSU_LDS_CASE(pid, T, E, ...) \
+  case pid:                           \
+    return FftPlan<T, E, __VA_ARGS__>::LDS;
+    GLSU_PLANS(GLSU_LDS_CASE)
+#undef GLSU_LDS_CASE
+    default:
+      return 0;
+  }
+}
+
 static int plan_smem_bytes(int id, int waves, bool ttab = false) {
-  // per transform: E rows of (T + 4) elements (the FftPlan::LDS layout), plus
-  // the shared E x T transition-twiddle table when the kernel keeps it in LDS
+  // per transform: the FftPlan::LDS exchange buffer, plus the shared E x T
+  // transition-twiddle table when the kernel keeps it in LDS
   for (auto& p : fft_plans())
     if (p.id == id)
-      return (int)sizeof(float2) * ((waves * 64 / p.T) * p.E * (p.T + 4) + (ttab ? p.E * p.T : 0));
+      return (int)sizeof(float2) * ((waves * 64 / p.T) * plan_lds(id) + (ttab ? p.E * p.T : 0));
   return 0;
 }
 

@@ -1,6 +1,8 @@
 // wff_plan.hip -- WFF2 plan set-up (window spectra, twiddles, buffers) and threshold.
 #include "wff_kernels.h"
 
+#include <cstdlib>
+
 namespace glsu {
 
 #define GLSU_PLAN_INFO(id, T, E, ...) {id, T, E, T * E, #__VA_ARGS__},
@@ -159,7 +161,17 @@ int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh,
   p.Nx = px->N; p.Ny = py->N;
   // keep dynamic LDS per workgroup within 64 KiB
   while (p.cols_waves > 1 && plan_smem_bytes(plany, p.cols_waves, true) > 65536) p.cols_waves /= 2;
-  while (p.rows_waves > 1 && plan_smem_bytes(planx, p.rows_waves) > 65536) p.rows_waves /= 2;
+  // rows: one wavefront per transform, up to kRowWavesMax waves per row
+  p.rows_waves = 1;
+  if (const char* e = std::getenv("GLSU_ROW_WAVES")) {  // tuning override
+    const int v = std::atoi(e);
+    if (v >= 1 && v <= kRowWavesMax) p.rows_waves = std::min(v, p.nwx);
+  }
+  if (const char* e = std::getenv("GLSU_COL_WAVES")) {  // tuning override
+    const int v = std::atoi(e);
+    if (v == 1 || v == 2 || v == 4) p.cols_waves = v;
+  }
+  while (p.rows_waves > 1 && plan_smem_bytes(planx, p.rows_waves) > 65536) --p.rows_waves;
   const int wb = (W + kTileCols - 1) / kTileCols;
   p.plane_elems = (size_t)wb * kTileCols * H;
   // spectra tables

@@ -6,21 +6,16 @@ namespace glsu {
 template <int MODE>
 static hipError_t launch_rows_inv_impl(const WffPlan& p, float2* out, const UpdateArgs& u,
                                        hipStream_t s) {
-  const int waves = p.rows_waves;
-  const int smem = plan_smem_bytes(p.planx, waves);
+  const int rw = p.rows_waves;
+  const int smem = plan_smem_bytes(p.planx, rw);
   switch (p.planx) {
-#define BODY                                                                                   \
-  {                                                                                            \
-    const int per = items_per_block<P>(waves);                                                 \
-    const int grid = (p.H + per - 1) / per;                                                    \
-    if (waves == 1)                                                                            \
-      hipLaunchKernelGGL((wff_rows_inv_kernel<P, 1, MODE>), dim3(grid), dim3(64), smem, s,     \
-                         p.d_D, p.d_Hx, p.d_twx, p.W, p.H, p.nwx, p.plane_elems, p.scale, out, \
-                         u);                                                                   \
-    else                                                                                       \
-      hipLaunchKernelGGL((wff_rows_inv_kernel<P, 2, MODE>), dim3(grid), dim3(128), smem, s,    \
-                         p.d_D, p.d_Hx, p.d_twx, p.W, p.H, p.nwx, p.plane_elems, p.scale, out, \
-                         u);                                                                   \
+#define BODY                                                                                 \
+  if constexpr (P::T == 64) {                                                                \
+    hipLaunchKernelGGL((wff_rows_inv_kernel<P, MODE>), dim3(p.H), dim3(64 * rw), smem, s,    \
+                       p.d_D, p.d_Hx, p.d_twx, p.W, p.H, p.nwx, p.plane_elems, p.scale, out, \
+                       u);                                                                   \
+  } else {                                                                                   \
+    return hipErrorInvalidValue;                                                             \
   }
     GLSU_PLANS(GLSU_DISPATCH_CASE)
 #undef BODY

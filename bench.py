@@ -45,6 +45,34 @@ def col_kernel_bytes(W, H, nwx):
     return 2 * nwx * W * H * 8
 
 
+def _opt(n):
+    """Smallest even 2^a 3^b 5^c 7^d >= n (the reference's padded FFT size)."""
+    m = max(n, 2)
+    while True:
+        k = m
+        for p in (2, 3, 5, 7):
+            while k % p == 0:
+                k //= p
+        if k == 1 and m % 2 == 0:
+            return m
+        m += 1
+
+
+def _traffic(W, H, T, tuning):
+    """HBM bytes per launch of the column kernel from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), when that summary was taken on this workload and tuning."""
+    path = os.path.join(ROOT, "profiles", f"wff_cols_{W}x{H}_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("tuning") != tuning or t.get("size") != [H, W]:
+        return None
+    return {"bytes": int(t["hbm_bytes_per_launch"]), "source": os.path.relpath(path, ROOT)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,6 +84,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget for the FP64 CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="library default plans instead of the autotuned ones")
+    ap.add_argument("--tuning", default=None,
+                    help="fixed tuning as planx,plany,cols_waves,rows_waves (skips the tuner)")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="run one plain copy kernel (torch.neg) over this many MiB of float32 "
+                         "before the timed region, as the HBM counter calibration for "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -78,8 +114,21 @@ def main():
     frames = glsu.gen_frames(W, H, T, seed, amp0, amp1, carrier, noise)
     phi0 = glsu.synth.phase(W, H, amp0, carrier)
     out = torch.empty((T, H, W), dtype=torch.float32, device="cuda")
-    d = glsu.DPRA(phi0, params=params)
+    tuning = None
+    if args.tuning:
+        px, py, cw, rw = (int(v) for v in args.tuning.split(","))
+        tuning = dict(planx=px, plany=py, cols_waves=cw, rows_waves=rw)
+    elif not args.no_tune:  # untimed: picks FFT plans and launch shapes for this size
+        tuning = glsu.autotune.tuning_only(glsu.autotune.tune(
+            W, H, params, force=True, log=lambda m: print(m, file=sys.stderr, flush=True)))
+    d = glsu.DPRA(phi0, params=params, tuning=tuning)
     graph = args.mode == "graph"
+    if args.calib_copy_mb:
+        src = torch.ones(args.calib_copy_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     for _ in range(args.warmup):
         d(frames, out=out, graph=graph)
@@ -92,24 +141,38 @@ def main():
     ms_per_step = dt * 1e3 / max(args.steps, 1)
     fps = args.steps * T / dt
 
-    # --- roofline of the dominant kernel: per-kernel HIP-event times ------------
-    dp = glsu.DPRA(phi0, params=params)
-    kms = dp.profile(frames)
-    info = glsu.WFT2(W, H, params=params).info()
+    # --- roofline of the dominant kernel (the column WFF2 kernel) ----------------
+    # HIP events on the handle's stream around back-to-back launches of that
+    # kernel on the benchmark's own buffers (after the timed region).
+    kms = d.profile(frames[:8])
+    col_ms = d.time_kernel(2, reps=20)
+    info = glsu.WFT2(W, H, params=params, tuning=tuning).info()
     nwx, nwy, Ny = info["nwx"], info["nwy"], info["ny"]
-    col_ms = kms[2] / T
     flops = col_kernel_flops(W, H, nwx, nwy, Ny)
     achieved = flops / (col_ms * 1e-3) / 1e12
     hbm = col_kernel_bytes(W, H, nwx) / (col_ms * 1e-3) / 1e9
+    # whole-chain HBM figure: B_alg = 28 HW + 8 Np (1 + P) bytes per frame, with
+    # Np the padded size of the reference's rule (even 2^a 3^b 5^c 7^d >= size + 2s)
+    np_ref = _opt(H + 2 * info["sy"]) * _opt(W + 2 * info["sx"])
+    b_alg = 28 * W * H + 8 * np_ref * (1 + nwx * nwy)
     roofline = {
         "bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
         "kernel": "wff_cols_kernel", "avg_ms": round(col_ms, 5),
-        "note": "FP32 vector-ALU bound FFT kernel priced against the FP32 peak",
+        "note": "FFT butterflies on the FP32 vector ALU (no dense contraction, so no MFMA); "
+                "priced against the FP32 peak, which MI355X's matrix and vector units share",
         "hbm_GBps": round(hbm, 1), "hbm_frac": round(hbm / PEAK_HBM_GBS, 4),
-        "kernel_ms_per_frame": {k: round(v / T, 5) for k, v in
+        "chain_hbm": {"B_alg_per_frame": b_alg, "GBps": round(b_alg * fps / 1e9, 1),
+                      "frac": round(b_alg * fps / 1e9 / PEAK_HBM_GBS, 4)},
+        "kernel_ms_per_frame": {k: round(v / 8, 5) for k, v in
                                 zip(["dpra_ls", "rows_fwd", "cols", "rows_inv_update"], kms)},
     }
+
+    traffic = _traffic(W, H, T, d.tuning())
+    if traffic is not None:
+        roofline["traffic"] = traffic["bytes"]
+        roofline["traffic_over_algorithmic"] = round(traffic["bytes"] / col_kernel_bytes(W, H, nwx), 3)
+        roofline["traffic_source"] = traffic["source"]
 
     line = {
         "metric": METRIC, "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1,
@@ -120,7 +183,8 @@ def main():
                                "3x3 DPRA, WFF2 sigma=10, wx,wy in [-0.2,0.2] step 0.1 "
                                "(25 pairs), thr=6*sqrt(mean|z|^2/3), rr=1",
                    "height": H, "width": W, "frames_per_step": T, "pairs": nwx * nwy,
-                   "fft_len": [info["nx"], info["ny"]], "mode": args.mode},
+                   "fft_len": [info["nx"], info["ny"]], "mode": args.mode,
+                   "tuning": d.tuning()},
         "roofline": roofline,
     }
 
@@ -141,7 +205,7 @@ def main():
             if time.perf_counter() - t0 > args.cpu_seconds:
                 break
         cdt = time.perf_counter() - t0
-        gpu = glsu.DPRA(phi0, params=params)(frames[:n].contiguous()).cpu().numpy()
+        gpu = glsu.DPRA(phi0, params=params, tuning=tuning)(frames[:n].contiguous()).cpu().numpy()
         errs = []
         for k in range(n):
             e = np.angle(np.exp(1j * (gpu[k].astype(np.float64) - cpu_out[k])))

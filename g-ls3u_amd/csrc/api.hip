@@ -176,11 +176,22 @@ static int validate(int W, int H, const glsu_wff_params* pr, int* planx, int* pl
   return GLSU_OK;
 }
 
-static int plan_common(WffPlan& p, int W, int H, const glsu_wff_params* pr, int planx, int plany) {
+// Host-only validation of a tuning request (plan ids are checked by validate()).
+static int validate_tuning(const glsu_tuning* t) {
+  if (t && !(t->cols_waves < 0 || t->cols_waves == 1 || t->cols_waves == 2 || t->cols_waves == 4))
+    return fail(GLSU_ERR_ARG, "cols_waves must be 1, 2 or 4 (got %d)", t->cols_waves);
+  if (t && !(t->rows_waves < 0 || (t->rows_waves >= 1 && t->rows_waves <= kRowWavesMax)))
+    return fail(GLSU_ERR_ARG, "rows_waves must be in 1..%d (got %d)", kRowWavesMax, t->rows_waves);
+  return GLSU_OK;
+}
+
+static int plan_common(WffPlan& p, int W, int H, const glsu_wff_params* pr, const glsu_tuning* t) {
+  int planx = t ? t->planx : -1, plany = t ? t->plany : -1;
   if (int rc = validate(W, H, pr, &planx, &plany)) return rc;
   std::string err;
   int rc = wff_plan_init(p, W, H, pr->sigma_x, pr->wxl, pr->wxh, pr->wxi, pr->sigma_y, pr->wyl,
-                         pr->wyh, pr->wyi, pr->thr, planx, plany, err);
+                         pr->wyh, pr->wyi, pr->thr, planx, plany, t ? t->cols_waves : -1,
+                         t ? t->rows_waves : -1, err);
   if (rc == -2) return fail(GLSU_ERR_ARG, "%s", err.c_str());
   if (rc) return fail(GLSU_ERR_DEVICE, "%s", err.c_str());
   return GLSU_OK;
@@ -196,20 +207,21 @@ struct glsu_wft2 {
   float2* d_out = nullptr;
 };
 
-extern "C" int glsu_wft2_create_plan(glsu_wft2** out, int W, int H, int type,
-                                     const glsu_wff_params* pr, void* stream, int planx,
-                                     int plany) {
+extern "C" int glsu_wft2_create_tuned(glsu_wft2** out, int W, int H, int type,
+                                      const glsu_wff_params* pr, void* stream,
+                                      const glsu_tuning* t) {
   if (!out || !pr) return fail(GLSU_ERR_ARG, "null argument");
   *out = nullptr;
   if (type == GLSU_WFR) return fail(GLSU_ERR_UNSUPPORTED, "WFR2 mode is not provided");
   if (type != GLSU_WFF) return fail(GLSU_ERR_ARG, "unknown WFT type %d", type);
   {
-    int px = planx, py = plany;
+    int px = t ? t->planx : -1, py = t ? t->plany : -1;
     if (int rc = validate(W, H, pr, &px, &py)) return rc;
+    if (int rc = validate_tuning(t)) return rc;
   }
   if (int rc = check_device()) return rc;
   auto* h = new glsu_wft2();
-  int rc = plan_common(h->p, W, H, pr, planx, plany);
+  int rc = plan_common(h->p, W, H, pr, t);
   if (rc) { delete h; return rc; }
   rc = make_stream(stream, &h->s, &h->own);
   if (rc) { wff_plan_free(h->p); delete h; return rc; }
@@ -221,7 +233,7 @@ extern "C" int glsu_wft2_create_plan(glsu_wft2** out, int W, int H, int type,
 
 extern "C" int glsu_wft2_create(glsu_wft2** out, int W, int H, int type, const glsu_wff_params* p,
                                 void* stream) {
-  return glsu_wft2_create_plan(out, W, H, type, p, stream, -1, -1);
+  return glsu_wft2_create_tuned(out, W, H, type, p, stream, nullptr);
 }
 
 static int wft2_enqueue(glsu_wft2* h, const float2* f, float2* out) {
@@ -276,6 +288,19 @@ static double read_thr(const WffPlan& p, hipStream_t s) {
   hipMemcpyAsync(&t, p.d_thr, sizeof(float), hipMemcpyDeviceToHost, s);
   hipStreamSynchronize(s);
   return t;
+}
+
+static void get_tuning(const WffPlan& p, glsu_tuning* t) {
+  t->planx = p.planx;
+  t->plany = p.plany;
+  t->cols_waves = p.cols_waves;
+  t->rows_waves = p.rows_waves;
+}
+
+extern "C" int glsu_wft2_get_tuning(glsu_wft2* h, glsu_tuning* t) {
+  if (!h || !t) return fail(GLSU_ERR_ARG, "null argument");
+  get_tuning(h->p, t);
+  return GLSU_OK;
 }
 
 extern "C" int glsu_wft2_info(glsu_wft2* h, int* nx, int* ny, int* sx, int* sy, int* nwx, int* nwy,
@@ -383,7 +408,7 @@ static float wrap_d(double v) { return (float)std::atan2(std::sin(v), std::cos(v
 
 template <typename PT>
 static int dpra_create_impl(glsu_dpra** out, const PT* phi0, int W, int H, int rr,
-                            const glsu_wff_params* pr, void* stream, int planx, int plany) {
+                            const glsu_wff_params* pr, void* stream, const glsu_tuning* t) {
   if (!out || !phi0) return fail(GLSU_ERR_ARG, "null argument");
   *out = nullptr;
   if (rr < 1) return fail(GLSU_ERR_ARG, "reference update rate must be >= 1");
@@ -393,13 +418,14 @@ static int dpra_create_impl(glsu_dpra** out, const PT* phi0, int W, int H, int r
     pr = &def;
   }
   {
-    int px = planx, py = plany;
+    int px = t ? t->planx : -1, py = t ? t->plany : -1;
     if (int rc = validate(W, H, pr, &px, &py)) return rc;
+    if (int rc = validate_tuning(t)) return rc;
   }
   if (int rc = check_device()) return rc;
   auto* h = new glsu_dpra();
   h->W = W; h->H = H; h->rr = rr;
-  int rc = plan_common(h->p, W, H, pr, planx, plany);
+  int rc = plan_common(h->p, W, H, pr, t);
   if (rc) { delete h; return rc; }
   rc = make_stream(stream, &h->s, &h->own);
   if (!rc) rc = dpra_alloc(h);
@@ -420,16 +446,16 @@ static int dpra_create_impl(glsu_dpra** out, const PT* phi0, int W, int H, int r
 
 extern "C" int glsu_dpra_create(glsu_dpra** out, const double* phi0, int W, int H, int rr,
                                 const glsu_wff_params* p, void* stream) {
-  return dpra_create_impl(out, phi0, W, H, rr, p, stream, -1, -1);
+  return dpra_create_impl(out, phi0, W, H, rr, p, stream, nullptr);
 }
 extern "C" int glsu_dpra_create_f32(glsu_dpra** out, const float* phi0, int W, int H, int rr,
                                     const glsu_wff_params* p, void* stream) {
-  return dpra_create_impl(out, phi0, W, H, rr, p, stream, -1, -1);
+  return dpra_create_impl(out, phi0, W, H, rr, p, stream, nullptr);
 }
-extern "C" int glsu_dpra_create_plan(glsu_dpra** out, const double* phi0, int W, int H, int rr,
-                                     const glsu_wff_params* p, void* stream, int planx,
-                                     int plany) {
-  return dpra_create_impl(out, phi0, W, H, rr, p, stream, planx, plany);
+extern "C" int glsu_dpra_create_tuned(glsu_dpra** out, const double* phi0, int W, int H, int rr,
+                                      const glsu_wff_params* p, void* stream,
+                                      const glsu_tuning* t) {
+  return dpra_create_impl(out, phi0, W, H, rr, p, stream, t);
 }
 
 // one frame: LS -> (threshold on first call) -> rows fwd -> columns -> rows inv + update
@@ -632,6 +658,35 @@ extern "C" int glsu_dpra_profile(glsu_dpra* h, const void* frames, int nframes, 
   return GLSU_OK;
 }
 
+extern "C" int glsu_dpra_time_kernel(glsu_dpra* h, int stage, int reps, double* ms) {
+  if (!h || !ms || reps < 1 || stage < 1 || stage > 3)
+    return fail(GLSU_ERR_ARG, "invalid argument (stage 1..3, reps >= 1)");
+  WffPlan& p = h->p;
+  if (!p.thr_ready) return fail(GLSU_ERR_ARG, "run one frame before timing a kernel");
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  // every stage below is idempotent on the handle's buffers (no state swap)
+  UpdateArgs u{h->phiref, h->dref_hi, h->dref_lo, h->phicurr, h->dphi_hi, h->dphi_lo, nullptr,
+               nullptr};
+  auto one = [&]() -> hipError_t {
+    if (stage == 1) return launch_rows_fwd(p, h->z, h->s);
+    if (stage == 2) return launch_cols(p, h->s);
+    return launch_rows_inv_update(p, u, h->s);
+  };
+  HIP_TRY(one());  // warm
+  HIP_TRY(hipEventRecord(e0, h->s));
+  for (int i = 0; i < reps; ++i) HIP_TRY(one());
+  HIP_TRY(hipEventRecord(e1, h->s));
+  HIP_TRY(hipEventSynchronize(e1));
+  float t = 0;
+  HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / reps;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return GLSU_OK;
+}
+
 extern "C" int glsu_dpra_get_state(glsu_dpra* h, float* phiref, float* dphiref) {
   if (!h) return fail(GLSU_ERR_ARG, "null handle");
   const size_t n = (size_t)h->W * h->H;
@@ -672,6 +727,12 @@ extern "C" int glsu_dpra_copy_z(glsu_dpra* h, void* d_z) {
 extern "C" int glsu_dpra_thr(glsu_dpra* h, double* thr) {
   if (!h || !thr) return fail(GLSU_ERR_ARG, "null argument");
   *thr = read_thr(h->p, h->s);
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_get_tuning(glsu_dpra* h, glsu_tuning* t) {
+  if (!h || !t) return fail(GLSU_ERR_ARG, "null argument");
+  get_tuning(h->p, t);
   return GLSU_OK;
 }
 

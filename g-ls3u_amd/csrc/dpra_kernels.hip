@@ -68,7 +68,10 @@ __device__ __forceinline__ void sincos_half(float x, float* s, float* c) {
   *c = fmaf(-x2, pc, 1.f);                     // 1 - x^2 (1/2! - x^2/4! + ...)
 }
 
-constexpr int DTX = 32, DTY = 8;
+// 64 x 8 pixel tiles (one pixel per thread, 256-byte rows), walked row-major
+// and XCD-contiguous so that halo rows shared by neighbouring tiles are
+// fetched from HBM once per L2.
+constexpr int DTX = 64, DTY = 8;
 
 template <typename InT>
 __global__ __launch_bounds__(DTX* DTY) void dpra_ls_kernel(const InT* __restrict__ I,
@@ -79,7 +82,10 @@ __global__ __launch_bounds__(DTX* DTY) void dpra_ls_kernel(const InT* __restrict
   __shared__ float sS[DTY + 2][DTX + 2];  // sin(phi/2)
   __shared__ float sC[DTY + 2][DTX + 2];  // cos(phi/2)
   const int tx = threadIdx.x % DTX, ty = threadIdx.x / DTX;
-  const int x0 = blockIdx.x * DTX - 1, y0 = blockIdx.y * DTY - 1;
+  const int nbx = (W + DTX - 1) / DTX;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int bx = tile % nbx, by = tile / nbx;
+  const int x0 = bx * DTX - 1, y0 = by * DTY - 1;
   for (int i = threadIdx.x; i < (DTX + 2) * (DTY + 2); i += DTX * DTY) {
     const int ly = i / (DTX + 2), lx = i % (DTX + 2);
     const int gx = x0 + lx, gy = y0 + ly;
@@ -93,7 +99,7 @@ __global__ __launch_bounds__(DTX* DTY) void dpra_ls_kernel(const InT* __restrict
     sC[ly][lx] = hc;
   }
   __syncthreads();
-  const int x = blockIdx.x * DTX + tx, y = blockIdx.y * DTY + ty;
+  const int x = bx * DTX + tx, y = by * DTY + ty;
   if (x >= W || y >= H) return;
 
   const float hsp = sS[ty + 1][tx + 1], hcp = sC[ty + 1][tx + 1];
@@ -184,7 +190,7 @@ __global__ __launch_bounds__(DTX* DTY) void dpra_ls_kernel(const InT* __restrict
 
 hipError_t launch_dpra_ls(const void* frame, int dtype, const float* phiref, float2* z, int W,
                           int H, int pitch, hipStream_t s) {
-  dim3 grid((W + DTX - 1) / DTX, (H + DTY - 1) / DTY), block(DTX * DTY);
+  dim3 grid(((W + DTX - 1) / DTX) * ((H + DTY - 1) / DTY)), block(DTX * DTY);
   switch (dtype) {
     case IN_U8:
       hipLaunchKernelGGL(dpra_ls_kernel<uint8_t>, grid, block, 0, s, (const uint8_t*)frame,

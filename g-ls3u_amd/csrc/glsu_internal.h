@@ -42,11 +42,21 @@ struct WffPlan {
   size_t plane_elems = 0;   // elements per tiled plane
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2;
+// remap so that each XCD walks a contiguous range of work items (lines shared
+// by neighbouring items stay in one L2).
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+  const int q = nblk >> 3, r = nblk & 7;
+  const int k = bid & 7, local = bid >> 3;
+  return k * q + (k < r ? k : r) + local;
+}
+
 constexpr int kTileCols = 8;  // columns per tile in the intermediate planes
+constexpr int kRowWavesMax = 4;  // row kernels: one wavefront per transform, <= 4 per row
 
 int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh, double wxi,
                   double sigy, double wyl, double wyh, double wyi, double thr, int planx,
-                  int plany, std::string& err);
+                  int plany, int cols_waves, int rows_waves, std::string& err);
 void wff_plan_free(WffPlan& p);
 
 // Kernel launchers (all asynchronous on `s`).

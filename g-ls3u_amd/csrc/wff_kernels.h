@@ -42,18 +42,6 @@ namespace glsu {
   X(13, 32, 12, 4, 4, 2)                 \
   X(14, 32, 20, 4, 4, 2)
 
-// -------------------------------------------------------- XCD-aware order
-// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
-// XCD walks a contiguous range of work items (shared L2 lines stay local).
-__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
-  const int q = nblk >> 3, r = nblk & 7;
-  const int k = bid & 7, local = bid >> 3;
-  return k * q + (k < r ? k : r) + local;
-}
-
-// row kernels: one wavefront per transform, up to this many waves per row
-constexpr int kRowWavesMax = 4;
-
 __device__ __forceinline__ size_t tile_index(int y, int x, int H) {
   return ((size_t)(x / kTileCols) * H + y) * kTileCols + (x % kTileCols);
 }

@@ -1,8 +1,6 @@
 // wff_plan.hip -- WFF2 plan set-up (window spectra, twiddles, buffers) and threshold.
 #include "wff_kernels.h"
 
-#include <cstdlib>
-
 namespace glsu {
 
 #define GLSU_PLAN_INFO(id, T, E, ...) {id, T, E, T * E, #__VA_ARGS__},
@@ -122,7 +120,7 @@ static hipError_t upload(V*& dptr, const std::vector<V>& h) {
 
 int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh, double wxi,
                   double sigy, double wyl, double wyh, double wyi, double thr, int planx,
-                  int plany, std::string& err) {
+                  int plany, int cols_waves, int rows_waves, std::string& err) {
   p = WffPlan{};
   p.W = W;
   p.H = H;
@@ -159,18 +157,12 @@ int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh,
   }
   p.planx = planx; p.plany = plany;
   p.Nx = px->N; p.Ny = py->N;
-  // keep dynamic LDS per workgroup within 64 KiB
-  while (p.cols_waves > 1 && plan_smem_bytes(plany, p.cols_waves, true) > 65536) p.cols_waves /= 2;
   // rows: one wavefront per transform, up to kRowWavesMax waves per row
   p.rows_waves = 1;
-  if (const char* e = std::getenv("GLSU_ROW_WAVES")) {  // tuning override
-    const int v = std::atoi(e);
-    if (v >= 1 && v <= kRowWavesMax) p.rows_waves = std::min(v, p.nwx);
-  }
-  if (const char* e = std::getenv("GLSU_COL_WAVES")) {  // tuning override
-    const int v = std::atoi(e);
-    if (v == 1 || v == 2 || v == 4) p.cols_waves = v;
-  }
+  if (rows_waves >= 1 && rows_waves <= kRowWavesMax) p.rows_waves = std::min(rows_waves, p.nwx);
+  if (cols_waves == 1 || cols_waves == 2 || cols_waves == 4) p.cols_waves = cols_waves;
+  // keep dynamic LDS per workgroup within 64 KiB
+  while (p.cols_waves > 1 && plan_smem_bytes(plany, p.cols_waves, true) > 65536) p.cols_waves /= 2;
   while (p.rows_waves > 1 && plan_smem_bytes(planx, p.rows_waves) > 65536) --p.rows_waves;
   const int wb = (W + kTileCols - 1) / kTileCols;
   p.plane_elems = (size_t)wb * kTileCols * H;

@@ -5,9 +5,9 @@ WFT2_CUDAF / WFT2_cpu) over the C ABI of libglsu.so.  Device memory is handed
 over as torch tensors (plumbing only); all computation runs in the library's
 HIP kernels.
 """
-from ._lib import (GlsuError, PaddingError, WffParams, build, check, lib, fft_plans,  # noqa: F401
+from ._lib import (GlsuError, PaddingError, WffParams, Tuning, build, check, lib, fft_plans,  # noqa: F401
                    WFF, WFR, U8, F32, F64, LIB_PATH)
 from .params import wff_params, dpra_reference_params, wft2_default_params  # noqa: F401
 from .wft2 import WFT2  # noqa: F401
 from .dpra import DPRA, gen_frames  # noqa: F401
-from . import synth  # noqa: F401
+from . import synth, autotune  # noqa: F401

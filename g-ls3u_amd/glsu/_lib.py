@@ -36,6 +36,35 @@ class PaddingError(GlsuError):
     """Padded FFT size out of range; the reference constructors throw int -1."""
 
 
+class Tuning(ctypes.Structure):
+    """glsu_tuning: plan ids and launch shapes (-1 = library default)."""
+    _fields_ = [("planx", ctypes.c_int), ("plany", ctypes.c_int),
+                ("cols_waves", ctypes.c_int), ("rows_waves", ctypes.c_int)]
+
+    @staticmethod
+    def make(planx=-1, plany=-1, cols_waves=-1, rows_waves=-1):
+        return Tuning(int(planx), int(plany), int(cols_waves), int(rows_waves))
+
+
+def tuning_arg(tuning=None, planx=-1, plany=-1):
+    """Tuning struct from a Tuning, a dict with any of its fields, or None
+    (planx / plany given separately override the dict)."""
+    if isinstance(tuning, Tuning):
+        t = Tuning.make(tuning.planx, tuning.plany, tuning.cols_waves, tuning.rows_waves)
+    else:
+        t = Tuning.make(**{k: v for k, v in (tuning or {}).items()
+                           if k in ("planx", "plany", "cols_waves", "rows_waves")})
+    if planx >= 0:
+        t.planx = int(planx)
+    if plany >= 0:
+        t.plany = int(plany)
+    return t
+
+
+def tuning_dict(t):
+    return dict(planx=t.planx, plany=t.plany, cols_waves=t.cols_waves, rows_waves=t.rows_waves)
+
+
 class WffParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("sigma_x", "wxl", "wxh", "wxi", "sigma_y", "wyl", "wyh", "wyi", "thr")]
@@ -60,6 +89,7 @@ def lib():
     PI, PD = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
     PV = ctypes.POINTER(ctypes.c_void_p)
     PP = ctypes.POINTER(WffParams)
+    PT = ctypes.POINTER(Tuning)
     PF = ctypes.c_void_p
     U64 = ctypes.c_uint64
     sig = {
@@ -69,14 +99,17 @@ def lib():
         "glsu_wff_params_default": (I, [PP]),
         "glsu_wff_params_dpra_reference": (I, [PP]),
         "glsu_wft2_create": (I, [PV, I, I, I, PP, V]),
-        "glsu_wft2_create_plan": (I, [PV, I, I, I, PP, V, I, I]),
+        "glsu_wft2_create_tuned": (I, [PV, I, I, I, PP, V, PT]),
+        "glsu_wft2_get_tuning": (I, [V, PT]),
         "glsu_wft2_compute": (I, [V, V, V, PD]),
         "glsu_wft2_compute_host": (I, [V, PF, PF, PD]),
         "glsu_wft2_info": (I, [V, PI, PI, PI, PI, PI, PI, PD]),
         "glsu_wft2_destroy": (I, [V]),
         "glsu_dpra_create": (I, [PV, PD, I, I, I, PP, V]),
         "glsu_dpra_create_f32": (I, [PV, PF, I, I, I, PP, V]),
-        "glsu_dpra_create_plan": (I, [PV, PD, I, I, I, PP, V, I, I]),
+        "glsu_dpra_create_tuned": (I, [PV, PD, I, I, I, PP, V, PT]),
+        "glsu_dpra_get_tuning": (I, [V, PT]),
+        "glsu_dpra_time_kernel": (I, [V, I, I, PD]),
         "glsu_dpra_frame": (I, [V, V, I, V, V, PD]),
         "glsu_dpra_frame_host": (I, [V, PF, I, PF, PD]),
         "glsu_dpra_update_ref": (I, [V]),
@@ -112,11 +145,13 @@ def exported_symbols():
     """Names declared in include/glsu.h (for the load/export test)."""
     return [
         "glsu_last_error", "glsu_version", "glsu_device_count", "glsu_wff_params_default",
-        "glsu_wff_params_dpra_reference", "glsu_wft2_create", "glsu_wft2_create_plan",
-        "glsu_wft2_compute", "glsu_wft2_compute_host", "glsu_wft2_info", "glsu_wft2_destroy",
-        "glsu_dpra_create", "glsu_dpra_create_f32", "glsu_dpra_create_plan", "glsu_dpra_frame",
+        "glsu_wff_params_dpra_reference", "glsu_wft2_create", "glsu_wft2_create_tuned",
+        "glsu_wft2_compute", "glsu_wft2_compute_host", "glsu_wft2_info", "glsu_wft2_get_tuning",
+        "glsu_wft2_destroy",
+        "glsu_dpra_create", "glsu_dpra_create_f32", "glsu_dpra_create_tuned", "glsu_dpra_get_tuning",
+        "glsu_dpra_frame",
         "glsu_dpra_frame_host", "glsu_dpra_update_ref", "glsu_dpra_run", "glsu_dpra_run_graph",
-        "glsu_dpra_profile", "glsu_dpra_get_state", "glsu_dpra_set_state", "glsu_dpra_copy_z", "glsu_dpra_thr",
+        "glsu_dpra_profile", "glsu_dpra_time_kernel", "glsu_dpra_get_state", "glsu_dpra_set_state", "glsu_dpra_copy_z", "glsu_dpra_thr",
         "glsu_dpra_destroy", "glsu_gen_frames", "glsu_fft_plans",
     ]
 

@@ -1,0 +1,134 @@
+"""Autotuner for the WFF2 plan and launch shapes.
+
+The reference picks its cuFFT sizes with the OPT table and has no launch
+tuning (WFT/WFT2_CUDAf.cu:61-120).  Here several FFT plans cover a padded
+length (any N >= size + half window is exact) and the row / column kernels
+can run with different wavefront counts, so the fastest combination is picked
+by timing on the device.  Every candidate computes the same result up to FP32
+rounding; the choice only affects speed.
+
+The search is staged because the kernels are independent:
+  1. columns: (plany, cols_waves), timed on the column kernel alone;
+  2. rows:    (planx, rows_waves), timed on the forward + inverse row kernels.
+Timings come from glsu_dpra_profile (HIP events between kernels) over a few
+synthetic frames of the requested size.  Results are cached per device, size
+and WFF parameters in a JSON file.
+"""
+import json
+import os
+
+import numpy as np
+import torch
+
+from ._lib import fft_plans
+from .dpra import DPRA, gen_frames
+from .params import wff_params
+from . import synth
+
+CACHE_ENV = "GLSU_TUNE_CACHE"
+_DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "glsu", "autotune.json")
+
+
+def _c_round(v):
+    return int(np.floor(abs(v) + 0.5)) * (1 if v >= 0 else -1)
+
+
+def candidates(need, t64_only, limit=3):
+    """FFT plan ids with N >= need, shortest first (at most `limit` distinct N)."""
+    c = sorted((p["N"], p["id"]) for p in fft_plans()
+               if p["N"] >= need and (p["T"] == 64 or not t64_only))
+    out, lengths = [], []
+    for n, i in c:
+        if n not in lengths:
+            if len(lengths) == limit:
+                break
+            lengths.append(n)
+        out.append(i)
+    return out
+
+
+def key(width, height, params, device=None):
+    dev = torch.cuda.get_device_name(device) if torch.cuda.is_available() else "none"
+    p = params
+    return (f"{dev}|{width}x{height}|{p.sigma_x:.6g},{p.wxl:.9g},{p.wxh:.9g},{p.wxi:.9g}|"
+            f"{p.sigma_y:.6g},{p.wyl:.9g},{p.wyh:.9g},{p.wyi:.9g}")
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save(path, table):
+    os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def _time(phi0, params, frames, tuning, reps):
+    d = DPRA(phi0, 1, params=params, tuning=tuning)
+    d.profile(frames[:1])  # warm-up: threshold, first launches
+    best = None
+    for _ in range(reps):
+        ms = d.profile(frames)
+        best = ms if best is None else [min(a, b) for a, b in zip(best, ms)]
+    used = d.tuning()
+    d.close()
+    return used, [m / frames.shape[0] for m in best]
+
+
+def tune(width, height, params=None, frames=4, reps=3, cache=None, force=False, log=None):
+    """Pick (planx, plany, cols_waves, rows_waves) for a width x height WFF2 /
+    DPRA with `params`.  Returns a dict usable as `tuning=` for glsu.DPRA and
+    glsu.WFT2, with the per-frame kernel times it measured (key "ms")."""
+    params = params if params is not None else wff_params()
+    path = cache or os.environ.get(CACHE_ENV, _DEFAULT_CACHE)
+    k = key(width, height, params)
+    table = _load(path) if path else {}
+    if not force and k in table:
+        return dict(table[k])
+    sx, sy = _c_round(3 * params.sigma_x), _c_round(3 * params.sigma_y)
+    phi0 = synth.phase(width, height, 3.0, 0.05)
+    fr = gen_frames(width, height, frames, seed=1234)
+    say = log or (lambda *a: None)
+
+    # 1. columns (row settings at their defaults)
+    best_c = None
+    for py in candidates(height + sy, False):
+        for cw in (1, 2, 4):
+            used, ms = _time(phi0, params, fr, dict(plany=py, cols_waves=cw), reps)
+            if used["cols_waves"] != cw:  # LDS limit lowered it: same as a smaller one
+                continue
+            say(f"cols plany={py} waves={cw}: {ms[2] * 1e3:.1f} us")
+            if best_c is None or ms[2] < best_c[0]:
+                best_c = (ms[2], py, cw)
+    # 2. rows
+    best_r = None
+    for px in candidates(width + sx, True):
+        for rw in (1, 2, 4):
+            used, ms = _time(phi0, params, fr, dict(planx=px, plany=best_c[1],
+                                                    cols_waves=best_c[2], rows_waves=rw), reps)
+            if used["rows_waves"] != rw:
+                continue
+            t = ms[1] + ms[3]
+            say(f"rows planx={px} waves={rw}: {t * 1e3:.1f} us")
+            if best_r is None or t < best_r[0]:
+                best_r = (t, px, rw)
+    tuning = dict(planx=best_r[1], plany=best_c[1], cols_waves=best_c[2], rows_waves=best_r[2])
+    _, ms = _time(phi0, params, fr, tuning, reps)
+    res = dict(tuning, ms=ms)
+    if path:
+        table = _load(path)
+        table[k] = res
+        _save(path, table)
+    return dict(res)
+
+
+def tuning_only(res):
+    """Strip measurement fields from a tune() result."""
+    return {k: res[k] for k in ("planx", "plany", "cols_waves", "rows_waves")}

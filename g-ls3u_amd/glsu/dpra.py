@@ -10,28 +10,30 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import F32, F64, U8, check, lib
+from ._lib import F32, F64, U8, Tuning, check, lib, tuning_arg, tuning_dict
 from .params import wff_params
 
 _DT = {torch.uint8: U8, torch.float32: F32, torch.float64: F64}
 
 
 class DPRA:
-    def __init__(self, phi0, rr=1, params=None, stream=None, planx=-1, plany=-1):
+    def __init__(self, phi0, rr=1, params=None, stream=None, planx=-1, plany=-1, tuning=None):
         """phi0: host array (H, W) of the initial phase (float64 recommended).
         params: WffParams; None -> sigma 10, [-0.2, 0.2] step 0.1, thr < 0
         (the configs' settings; glsu.dpra_reference_params() gives the values
-        the reference DPRA hard-codes)."""
+        the reference DPRA hard-codes).  tuning: glsu.Tuning or a dict of its
+        fields (plan ids, launch shapes), e.g. from glsu.autotune."""
         phi = np.ascontiguousarray(np.asarray(phi0, dtype=np.float64))
         self.height, self.width = phi.shape
         self.rr = int(rr)
         self.params = params if params is not None else wff_params()
         self.stream = stream if stream is not None else torch.cuda.current_stream()
         h = ctypes.c_void_p()
-        check(lib().glsu_dpra_create_plan(
+        t = tuning_arg(tuning, planx, plany)
+        check(lib().glsu_dpra_create_tuned(
             ctypes.byref(h), phi.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), self.width,
             self.height, self.rr, ctypes.byref(self.params), ctypes.c_void_p(self.stream.cuda_stream),
-            int(planx), int(plany)))
+            ctypes.byref(t)))
         self._h = h
         self.last_ms = None
 
@@ -49,6 +51,12 @@ class DPRA:
         lib().glsu_dpra_destroy(h)
 
     __del__ = close
+
+    def tuning(self):
+        """The plan ids and launch shapes in use."""
+        t = Tuning.make()
+        check(lib().glsu_dpra_get_tuning(self._h, ctypes.byref(t)))
+        return tuning_dict(t)
 
     def _check_frame(self, frame, nd=2):
         if not frame.is_cuda or not frame.is_contiguous() or frame.dtype not in _DT:
@@ -98,6 +106,13 @@ class DPRA:
         check(lib().glsu_dpra_profile(self._h, ctypes.c_void_p(frames.data_ptr()), int(T), dt,
                                       ctypes.c_void_p(out.data_ptr()), ms))
         return list(ms)
+
+    def time_kernel(self, stage, reps=20):
+        """Average ms per launch of one WFF2 kernel (1 rows fwd, 2 columns,
+        3 rows inv + update), timed with HIP events on the handle's stream."""
+        ms = ctypes.c_double()
+        check(lib().glsu_dpra_time_kernel(self._h, int(stage), int(reps), ctypes.byref(ms)))
+        return ms.value
 
     def get_state(self):
         n = self.height * self.width

@@ -9,22 +9,24 @@ import ctypes
 import torch
 
 from . import _lib
-from ._lib import WFF, check, lib
+from ._lib import WFF, Tuning, check, lib, tuning_arg, tuning_dict
 from .params import wff_params
 
 
 class WFT2:
-    def __init__(self, width, height, type=WFF, params=None, stream=None, planx=-1, plany=-1):
+    def __init__(self, width, height, type=WFF, params=None, stream=None, planx=-1, plany=-1, tuning=None):
         """params: WffParams (see glsu.wff_params); None -> wff_params() defaults
-        of the configs (sigma 10, [-0.2, 0.2] step 0.1, thr < 0)."""
+        (sigma 10, [-0.2, 0.2] step 0.1, thr < 0).  tuning: glsu.Tuning or a
+        dict of its fields (plan ids, launch shapes), e.g. from glsu.autotune."""
         self.width, self.height = int(width), int(height)
         self.params = params if params is not None else wff_params()
         self.stream = stream if stream is not None else torch.cuda.current_stream()
         h = ctypes.c_void_p()
-        check(lib().glsu_wft2_create_plan(ctypes.byref(h), self.width, self.height, int(type),
-                                          ctypes.byref(self.params),
-                                          ctypes.c_void_p(self.stream.cuda_stream),
-                                          int(planx), int(plany)))
+        t = tuning_arg(tuning, planx, plany)
+        check(lib().glsu_wft2_create_tuned(ctypes.byref(h), self.width, self.height, int(type),
+                                           ctypes.byref(self.params),
+                                           ctypes.c_void_p(self.stream.cuda_stream),
+                                           ctypes.byref(t)))
         self._h = h
         self.last_ms = None
 
@@ -49,6 +51,12 @@ class WFT2:
         check(lib().glsu_wft2_info(self._h, *[ctypes.byref(x) for x in v], ctypes.byref(thr)))
         nx, ny, sx, sy, nwx, nwy = (x.value for x in v)
         return dict(nx=nx, ny=ny, sx=sx, sy=sy, nwx=nwx, nwy=nwy, thr=thr.value)
+
+    def tuning(self):
+        """The plan ids and launch shapes in use."""
+        t = Tuning.make()
+        check(lib().glsu_wft2_get_tuning(self._h, ctypes.byref(t)))
+        return tuning_dict(t)
 
     def __call__(self, f, out=None, timed=False):
         """f: complex64 CUDA tensor (height, width).  Returns the filtered field."""

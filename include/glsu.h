@@ -46,6 +46,16 @@ typedef struct glsu_wff_params {
   double thr;
 } glsu_wff_params;
 
+/* Launch/plan choices picked by the autotuner (g-ls3u_amd/glsu/autotune.py).
+ * Any field < 0 keeps the library default.  planx / plany: FFT plan ids
+ * (glsu_fft_plans) along rows and columns; cols_waves: wavefronts per
+ * workgroup of the column kernel (1, 2 or 4); rows_waves: wavefronts per row
+ * of the row kernels (1..4).  Results do not depend on these choices beyond
+ * FP32 rounding.  No reference counterpart. */
+typedef struct glsu_tuning {
+  int planx, plany, cols_waves, rows_waves;
+} glsu_tuning;
+
 typedef struct glsu_wft2 glsu_wft2;
 typedef struct glsu_dpra glsu_dpra;
 
@@ -72,11 +82,10 @@ int glsu_wff_params_dpra_reference(glsu_wff_params *p);
  * WFT/WFT2_CPU.cpp:69-73), GLSU_ERR_UNSUPPORTED for type == GLSU_WFR. */
 int glsu_wft2_create(glsu_wft2 **out, int width, int height, int type,
                      const glsu_wff_params *p, void *hip_stream);
-/* Same, with explicit FFT plan ids along x and y (-1 = automatic); used by the
- * autotuner.  No counterpart in the reference. */
-int glsu_wft2_create_plan(glsu_wft2 **out, int width, int height, int type,
-                          const glsu_wff_params *p, void *hip_stream, int planx,
-                          int plany);
+/* Same, with explicit tuning (NULL = defaults); used by the autotuner.
+ * No counterpart in the reference. */
+int glsu_wft2_create_tuned(glsu_wft2 **out, int width, int height, int type,
+                           const glsu_wff_params *p, void *hip_stream, const glsu_tuning *t);
 /* Replaces WFT2_CUDAF::operator()(cufftComplex *d_f, WFT2_DeviceResultsF &d_z,
  * double &time), WFT/WFT2_CUDAf.h:38-40 -> cuWFF2, WFT/WFT2_CUDAf.cu:1016-1173.
  * d_f, d_filtered: device arrays of height*width complex64 (interleaved
@@ -89,6 +98,9 @@ int glsu_wft2_compute_host(glsu_wft2 *h, const float *h_f, float *h_filtered, do
 /* info: padded FFT lengths (this library's, not the reference's OPT sizes),
  * half window sizes, grid sizes, and the threshold in effect (NaN until the
  * first call when thr < 0). */
+/* The plan ids and launch shapes in use (after defaults and LDS limits are
+ * applied).  No counterpart in the reference. */
+int glsu_wft2_get_tuning(glsu_wft2 *h, glsu_tuning *t);
 int glsu_wft2_info(glsu_wft2 *h, int *nx, int *ny, int *sx, int *sy, int *nwx, int *nwy,
                    double *thr);
 int glsu_wft2_destroy(glsu_wft2 *h);
@@ -105,9 +117,9 @@ int glsu_dpra_create(glsu_dpra **out, const double *phi0, int width, int height,
                      const glsu_wff_params *p, void *hip_stream);
 int glsu_dpra_create_f32(glsu_dpra **out, const float *phi0, int width, int height, int rr,
                          const glsu_wff_params *p, void *hip_stream);
-/* Same, with explicit FFT plan ids (-1 = automatic).  No reference counterpart. */
-int glsu_dpra_create_plan(glsu_dpra **out, const double *phi0, int width, int height, int rr,
-                          const glsu_wff_params *p, void *hip_stream, int planx, int plany);
+/* Same, with explicit tuning (NULL = defaults).  No reference counterpart. */
+int glsu_dpra_create_tuned(glsu_dpra **out, const double *phi0, int width, int height, int rr,
+                           const glsu_wff_params *p, void *hip_stream, const glsu_tuning *t);
 /* Replaces DPRA_CUDAF::dpra_per_frame(const cv::Mat &img, std::vector<float> &dPhi,
  * double &time), DPRA/dpra_cudaf.h:41-43, DPRA/dpra_cudaf.cu:207-284 (FP64 twin
  * DPRA/dpra_cpu.cpp:130-252).  d_frame: DEVICE array of height*width elements of
@@ -141,6 +153,12 @@ int glsu_dpra_run_graph(glsu_dpra *h, const void *d_frames, int nframes, int fra
  * DPRA/dpra_cudaf.cu:219-283). */
 int glsu_dpra_profile(glsu_dpra *h, const void *d_frames, int nframes, int frame_dtype,
                       float *d_dphi_all, double *ms);
+/* Diagnostics: average device time (ms) of one launch of a WFF2 kernel, from
+ * `reps` back-to-back launches between two HIP events on the handle's stream.
+ * stage: 1 row-forward, 2 column, 3 row-inverse+update.  Re-runs the kernel on
+ * the handle's current buffers (idempotent; call after at least one frame).
+ * No reference counterpart. */
+int glsu_dpra_time_kernel(glsu_dpra *h, int stage, int reps, double *ms);
 /* State access for checkpoint/resume (the reference cannot seed dphiref,
  * DPRA/dpra_cpu.cpp:19).  Host float arrays of height*width; any may be NULL. */
 int glsu_dpra_get_state(glsu_dpra *h, float *phiref, float *dphiref);
@@ -150,6 +168,8 @@ int glsu_dpra_set_state(glsu_dpra *h, const float *phiref, const float *dphiref)
 int glsu_dpra_copy_z(glsu_dpra *h, void *d_z);
 /* the WFF threshold in effect (NaN before the first frame when thr < 0) */
 int glsu_dpra_thr(glsu_dpra *h, double *thr);
+/* The plan ids and launch shapes in use.  No counterpart in the reference. */
+int glsu_dpra_get_tuning(glsu_dpra *h, glsu_tuning *t);
 int glsu_dpra_destroy(glsu_dpra *h);
 
 /* ---- utilities ---------------------------------------------------------- */

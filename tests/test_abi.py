@@ -76,6 +76,38 @@ def test_argument_errors():
     assert glsu.lib().glsu_dpra_frame(None, None, 1, None, None, None) == _lib.ERR_ARG
 
 
+def test_tuning_errors():
+    # tuning requests are checked on the host, before any device work
+    p = glsu.wff_params()
+    h = ctypes.c_void_p()
+
+    def create(**kw):
+        t = _lib.tuning_arg(kw)
+        return glsu.lib().glsu_wft2_create_tuned(ctypes.byref(h), 200, 150, 0, ctypes.byref(p),
+                                                 None, ctypes.byref(t))
+    assert create(cols_waves=3) == _lib.ERR_ARG
+    assert create(rows_waves=5) == _lib.ERR_ARG
+    assert create(rows_waves=0) == _lib.ERR_ARG
+    plans = {q["id"]: q for q in glsu.fft_plans()}
+    narrow = [i for i, q in plans.items() if q["T"] != 64 and q["N"] >= 200 + 30]
+    short = [i for i, q in plans.items() if q["N"] < 150]
+    assert create(planx=narrow[0]) == _lib.ERR_ARG      # x axis needs a T == 64 plan
+    assert create(plany=short[0]) == _lib.ERR_ARG       # too short for H + half window
+    assert create(planx=999) == _lib.ERR_ARG
+
+
+def test_autotune_candidates():
+    from glsu import autotune
+    plans = {q["id"]: q for q in glsu.fft_plans()}
+    c = autotune.candidates(1054, True)
+    assert c and all(plans[i]["T"] == 64 and plans[i]["N"] >= 1054 for i in c)
+    assert len({plans[i]["N"] for i in c}) <= 3
+    assert plans[c[0]]["N"] == min(q["N"] for q in plans.values()
+                                   if q["T"] == 64 and q["N"] >= 1054)
+    c2 = autotune.candidates(300, False)
+    assert any(plans[i]["T"] != 64 for i in c2)
+
+
 def test_no_device_is_reported_not_faked():
     import torch
     if torch.cuda.is_available():

@@ -30,6 +30,15 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
+# name -> (size, frames, WFF2 (sigma, lo, hi, step), generator (amp0, amp1, carrier, noise))
+WORKLOADS = {
+    # the headline configuration: 1024^2, 200 frames, 25 pairs
+    "plain1024": (1024, 200, (10.0, -0.2, 0.2, 0.1), (3.0, 0.1, 0.05, 10.0)),
+    # heavy WFF2: 2048^2, 50 frames, 441 pairs, noise 20, carrier 0.08
+    "heavy2048": (2048, 50, (12.0, -0.5, 0.5, 0.05), (3.0, 0.1, 0.08, 20.0)),
+}
+
+
 def col_kernel_flops(W, H, nwx, nwy, Ny):
     """Algorithmic FLOP of one launch of the column kernel: per (wx, column)
     item, 2 + 2*nwy complex FFTs of length Ny (5 N log2 N each) plus per pair
@@ -78,8 +87,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--size", type=int, default=1024)
-    ap.add_argument("--frames", type=int, default=200)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="plain1024")
+    ap.add_argument("--size", type=int, default=None, help="override the workload's size")
+    ap.add_argument("--frames", type=int, default=None, help="override the workload's frames")
     ap.add_argument("--mode", choices=["graph", "eager"], default="graph")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget for the FP64 CPU baseline sample")
@@ -105,11 +115,12 @@ def main():
     import glsu
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    W = H = args.size
-    T = args.frames
+    size, T, wff, gen = WORKLOADS[args.workload]
+    W = H = args.size or size
+    T = args.frames or T
     seed = 20240607
-    amp0, amp1, carrier, noise = 3.0, 0.1, 0.05, 10.0
-    params = glsu.wff_params(10.0, -0.2, 0.2, 0.1)
+    amp0, amp1, carrier, noise = gen
+    params = glsu.wff_params(*wff)
 
     frames = glsu.gen_frames(W, H, T, seed, amp0, amp1, carrier, noise)
     phi0 = glsu.synth.phase(W, H, amp0, carrier)
@@ -180,8 +191,10 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"plain FP32 DPRA+WFF2 chain, {W}x{H}, {T} frames per step, "
-                               "3x3 DPRA, WFF2 sigma=10, wx,wy in [-0.2,0.2] step 0.1 "
-                               "(25 pairs), thr=6*sqrt(mean|z|^2/3), rr=1",
+                               f"3x3 DPRA, WFF2 sigma={wff[0]:g}, wx,wy in [{wff[1]:g},{wff[2]:g}] "
+                               f"step {wff[3]:g} ({nwx * nwy} pairs), thr=6*sqrt(mean|z|^2/3), "
+                               f"rr=1, noise {noise:g}, carrier {carrier:g}",
+                   "name": args.workload,
                    "height": H, "width": W, "frames_per_step": T, "pairs": nwx * nwy,
                    "fft_len": [info["nx"], info["ny"]], "mode": args.mode,
                    "tuning": d.tuning()},
@@ -193,7 +206,7 @@ def main():
         from oracle import oracle as O
         nthreads = min(os.cpu_count() or 1, 16)
         host = frames.cpu().numpy().astype(np.float64)
-        od = O.DPRA(phi0, nthreads=nthreads)
+        od = O.DPRA(phi0, sigx=wff[0], wxl=wff[1], wxh=wff[2], wxi=wff[3], nthreads=nthreads)
         cpu_out = []
         t0 = time.perf_counter()
         n = 0

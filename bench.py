@@ -36,7 +36,85 @@ WORKLOADS = {
     "plain1024": (1024, 200, (10.0, -0.2, 0.2, 0.1), (3.0, 0.1, 0.05, 10.0)),
     # heavy WFF2: 2048^2, 50 frames, 441 pairs, noise 20, carrier 0.08
     "heavy2048": (2048, 50, (12.0, -0.5, 0.5, 0.05), (3.0, 0.1, 0.08, 20.0)),
+    # 64 sequences, sizes uniform over {256..2048}^2, 30 frames each, batched
+    "batch64": (None, 30, (10.0, -0.2, 0.2, 0.1), None),
 }
+BATCH_SIZES = [256, 512, 768, 1024, 1536, 2048]
+
+
+def bench_batch(args, glsu, torch, np):
+    """64 independent sequences of uneven sizes in one batched run per step
+    (persistent launches, one graph replay per frame).  value = frames/s over
+    all sequences; also reported: the same sequences run one after another
+    through single-sequence handles."""
+    _, T, wff, _ = WORKLOADS["batch64"]
+    T = args.frames or T
+    n = 64
+    rng = np.random.default_rng(20240607)
+    sizes = [int(v) for v in rng.choice(BATCH_SIZES, n)]
+    amp0 = rng.uniform(2.0, 5.0, n)
+    noise = rng.uniform(5.0, 25.0, n)
+    params = glsu.wff_params(*wff)
+    frames, phis = [], []
+    for i, S in enumerate(sizes):
+        frames.append(glsu.gen_frames(S, S, T, 1000 + i, amp0=float(amp0[i]), amp1=0.1,
+                                      carrier=0.05, noise=float(noise[i])))
+        phis.append(glsu.synth.phase(S, S, float(amp0[i]), 0.05))
+    tuned = {}
+    if not args.no_tune:
+        for S in sorted(set(sizes)):
+            tuned[S] = glsu.autotune.tuning_only(glsu.autotune.tune(
+                S, S, params, force=True, log=lambda m: print(m, file=sys.stderr, flush=True)))
+    tunings = [tuned.get(S) for S in sizes] if tuned else None
+    b = glsu.DPRABatch(phis, 1, params=params, tunings=tunings)
+    outs = [torch.empty((T, S, S), dtype=torch.float32, device="cuda") for S in sizes]
+    graph = args.mode == "graph"
+    for _ in range(args.warmup):
+        b(frames, outs=outs, graph=graph)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b(frames, outs=outs, graph=graph)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    fps = args.steps * n * T / dt
+    # the same work through one single-sequence handle per sequence, in turn
+    singles = [glsu.DPRA(phis[i], 1, params=params, tuning=tunings[i] if tunings else None)
+               for i in range(n)]
+    for i in range(n):
+        singles[i](frames[i], out=outs[i], graph=graph)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(n):
+        singles[i](frames[i], out=outs[i], graph=graph)
+    torch.cuda.synchronize()
+    seq_fps = n * T / (time.perf_counter() - t1)
+    px = sum(S * S for S in sizes)
+    np_ref = sum(_opt(S + 60) ** 2 for S in sizes)
+    b_alg = 28 * px + 8 * np_ref * 26
+    line = {
+        "metric": METRIC, "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / max(args.steps, 1), 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batched FP32 DPRA+WFF2: {n} sequences, sizes uniform over "
+                               f"{{256..2048}}^2, {T} frames each, one batched run per step, "
+                               "WFF2 sigma=10, [-0.2,0.2] step 0.1 (25 pairs), per-sequence "
+                               "amp0 in U[2,5] (+0.1 t) and noise in U[5,25]",
+                   "name": "batch64", "sizes": sizes, "plan_classes": b.info()["classes"],
+                   "mode": args.mode},
+        "sequential_single_handles_frames_per_s": round(seq_fps, 2),
+        "batched_over_sequential": round(fps / seq_fps, 3),
+        "megapixels_per_s": round(args.steps * T * px / dt / 1e6, 1),
+        "roofline": {"bound": "hbm", "chain_B_alg_per_step_frame": b_alg,
+                     "achieved": round(b_alg * (fps / n) / 1e9, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(b_alg * (fps / n) / 1e9 / PEAK_HBM_GBS, 4),
+                     "traffic": None,
+                     "note": "whole-chain algorithmic bytes (28 HW + 8 Np (1+P) per frame, "
+                             "reference padding) over all sequences per step; per-kernel "
+                             "figures are those of the plain1024 workload"},
+    }
+    print(json.dumps(line), flush=True)
 
 
 def col_kernel_flops(W, H, nwx, nwy, Ny):
@@ -115,6 +193,8 @@ def main():
     import glsu
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.workload == "batch64":
+        return bench_batch(args, glsu, torch, np)
     size, T, wff, gen = WORKLOADS[args.workload]
     W = H = args.size or size
     T = args.frames or T

@@ -752,3 +752,324 @@ extern "C" int glsu_gen_frames(float* d_out, int W, int H, int t0, int T, uint64
                             (hipStream_t)stream));
   return GLSU_OK;
 }
+
+// ============================================================ DPRA batch
+// Many independent sequences of uneven sizes in one set of persistent
+// launches per frame: one least-squares launch over every image, and per FFT
+// plan class (images sharing planx/plany/launch shapes) one launch per WFF2
+// stage, all driven by a device table of images and a device frame counter,
+// so one captured graph serves every frame.  Each image is computed by the
+// same per-item code as a single-sequence handle with the same tuning, so the
+// results are bit-identical to glsu_dpra_run on that handle.
+namespace {
+struct BatchImage {
+  int W = 0, H = 0, rr = 1;
+  float *phi[2] = {nullptr, nullptr}, *dhi[2] = {nullptr, nullptr}, *dlo[2] = {nullptr, nullptr};
+  float2 *z = nullptr, *A = nullptr, *D = nullptr;
+  size_t plane = 0;
+  float *thr = nullptr, *red = nullptr;
+  int seq = 0;  // index in the caller's list
+};
+struct BatchCls {
+  WffPlan p;  // tables only (spectra, twiddles), plan ids and launch shapes
+  int b = 0, e = 0;
+  BatchClass bc{};
+};
+}  // namespace
+
+struct glsu_dpra_batch {
+  int n = 0;
+  hipStream_t s = nullptr, cap = nullptr;
+  bool own = false;
+  std::vector<BatchImage> img;  // table order
+  std::vector<int> pos;         // caller index -> table index
+  std::vector<BatchCls> cls;
+  BatchSeg* d_tab = nullptr;
+  BatchSeg* h_tab = nullptr;    // pinned staging of the table
+  int* d_frame = nullptr;
+  int* h_frame = nullptr;       // pinned
+  int ntiles = 0, grid_ls = 0;
+  std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;  // by frame dtype
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+static void batch_free(glsu_dpra_batch* h) {
+  if (h->s) hipStreamSynchronize(h->s);
+  for (auto& kv : h->graphs) {
+    hipGraphExecDestroy(kv.second.second);
+    hipGraphDestroy(kv.second.first);
+  }
+  for (auto& im : h->img) {
+    for (int k = 0; k < 2; ++k) {
+      hipFree(im.phi[k]); hipFree(im.dhi[k]); hipFree(im.dlo[k]);
+    }
+    hipFree(im.z); hipFree(im.A); hipFree(im.D); hipFree(im.thr); hipFree(im.red);
+  }
+  for (auto& c : h->cls) wff_plan_free(c.p);
+  hipFree(h->d_tab); hipFree(h->d_frame);
+  if (h->h_tab) hipHostFree(h->h_tab);
+  if (h->h_frame) hipHostFree(h->h_frame);
+  if (h->e0) hipEventDestroy(h->e0);
+  if (h->e1) hipEventDestroy(h->e1);
+  if (h->own && h->s) hipStreamDestroy(h->s);
+  if (h->cap) hipStreamDestroy(h->cap);
+}
+
+static int batch_alloc_image(BatchImage& im, const WffPlan& cp, const double* phi0, double thr) {
+  const size_t n = (size_t)im.W * im.H;
+  const int wb = (im.W + kTileCols - 1) / kTileCols;
+  im.plane = (size_t)wb * kTileCols * im.H;
+  for (int k = 0; k < 2; ++k) {
+    HIP_TRY(hipMalloc((void**)&im.phi[k], n * sizeof(float)));
+    HIP_TRY(hipMalloc((void**)&im.dhi[k], n * sizeof(float)));
+    HIP_TRY(hipMalloc((void**)&im.dlo[k], n * sizeof(float)));
+    HIP_TRY(hipMemset(im.dhi[k], 0, n * sizeof(float)));
+    HIP_TRY(hipMemset(im.dlo[k], 0, n * sizeof(float)));
+  }
+  HIP_TRY(hipMemset(im.phi[1], 0, n * sizeof(float)));
+  HIP_TRY(hipMalloc((void**)&im.z, n * sizeof(float2)));
+  HIP_TRY(hipMalloc((void**)&im.A, im.plane * cp.nwx * sizeof(float2)));
+  HIP_TRY(hipMalloc((void**)&im.D, im.plane * cp.nwx * sizeof(float2)));
+  HIP_TRY(hipMalloc((void**)&im.thr, sizeof(float)));
+  HIP_TRY(hipMalloc((void**)&im.red, 256 * sizeof(float)));
+  const float t0 = thr >= 0 ? (float)thr : -1.f;
+  HIP_TRY(hipMemcpy(im.thr, &t0, sizeof(float), hipMemcpyHostToDevice));
+  std::vector<float> ph(n);
+  for (size_t i = 0; i < n; ++i) ph[i] = wrap_d(phi0[i]);
+  HIP_TRY(hipMemcpy(im.phi[0], ph.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_batch_create(glsu_dpra_batch** out, const glsu_seq_desc* seqs, int n,
+                                      const glsu_wff_params* pr, void* stream,
+                                      const glsu_tuning* tunings) {
+  if (!out || !seqs || n < 1) return fail(GLSU_ERR_ARG, "invalid argument");
+  *out = nullptr;
+  glsu_wff_params def;
+  if (!pr) {
+    glsu_wff_params_dpra_reference(&def);
+    pr = &def;
+  }
+  // host-side validation of every sequence and its plan choice
+  struct Req { int px, py, cw, rw; };
+  std::vector<Req> req(n);
+  for (int i = 0; i < n; ++i) {
+    const glsu_seq_desc& d = seqs[i];
+    if (!d.phi0) return fail(GLSU_ERR_ARG, "sequence %d: null phi0", i);
+    if (d.rr < 1) return fail(GLSU_ERR_ARG, "sequence %d: reference update rate must be >= 1", i);
+    const glsu_tuning* t = tunings ? &tunings[i] : nullptr;
+    int px = t ? t->planx : -1, py = t ? t->plany : -1;
+    if (int rc = validate(d.width, d.height, pr, &px, &py)) return rc;
+    if (int rc = validate_tuning(t)) return rc;
+    req[i] = {px, py, t ? t->cols_waves : -1, t ? t->rows_waves : -1};
+  }
+  if (int rc = check_device()) return rc;
+  auto* h = new glsu_dpra_batch();
+  h->n = n;
+  auto bail = [&](int rc) {
+    batch_free(h);
+    delete h;
+    return rc;
+  };
+  if (int rc = make_stream(stream, &h->s, &h->own)) return bail(rc);
+  // plan classes, in order of first appearance
+  std::vector<int> cls_of(n, -1);
+  std::vector<Req> keys;
+  for (int i = 0; i < n; ++i) {
+    int c = -1;
+    for (size_t k = 0; k < keys.size(); ++k)
+      if (keys[k].px == req[i].px && keys[k].py == req[i].py && keys[k].cw == req[i].cw &&
+          keys[k].rw == req[i].rw)
+        c = (int)k;
+    if (c < 0) {
+      c = (int)keys.size();
+      keys.push_back(req[i]);
+    }
+    cls_of[i] = c;
+  }
+  h->cls.resize(keys.size());
+  h->pos.assign(n, -1);
+  for (size_t c = 0; c < keys.size(); ++c) {
+    BatchCls& bc = h->cls[c];
+    int first = -1;
+    for (int i = 0; i < n; ++i)
+      if (cls_of[i] == (int)c && first < 0) first = i;
+    std::string err;
+    int rc = wff_plan_init(bc.p, seqs[first].width, seqs[first].height, pr->sigma_x, pr->wxl,
+                           pr->wxh, pr->wxi, pr->sigma_y, pr->wyl, pr->wyh, pr->wyi, pr->thr,
+                           keys[c].px, keys[c].py, keys[c].cw, keys[c].rw, err, false);
+    if (rc) return bail(fail(rc == -2 ? GLSU_ERR_ARG : GLSU_ERR_DEVICE, "%s", err.c_str()));
+    bc.b = (int)h->img.size();
+    for (int i = 0; i < n; ++i) {
+      if (cls_of[i] != (int)c) continue;
+      BatchImage im;
+      im.W = seqs[i].width; im.H = seqs[i].height; im.rr = seqs[i].rr; im.seq = i;
+      h->pos[i] = (int)h->img.size();
+      h->img.push_back(im);
+      if (int e = batch_alloc_image(h->img.back(), bc.p, seqs[i].phi0, pr->thr)) return bail(e);
+    }
+    bc.e = (int)h->img.size();
+  }
+  // table offsets and launch geometry
+  const int nt = (int)h->img.size();
+  if (hipHostMalloc((void**)&h->h_tab, sizeof(BatchSeg) * nt) != hipSuccess ||
+      hipHostMalloc((void**)&h->h_frame, sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&h->d_tab, sizeof(BatchSeg) * nt) != hipSuccess ||
+      hipMalloc((void**)&h->d_frame, sizeof(int)) != hipSuccess)
+    return bail(fail(GLSU_ERR_DEVICE, "batch table allocation failed"));
+  int tile = 0;
+  for (auto& c : h->cls) {
+    const WffPlan& p = c.p;
+    const int ipb = cols_items_per_block(p.plany, p.cols_waves);
+    int row = 0, blk = 0;
+    for (int k = c.b; k < c.e; ++k) {
+      BatchSeg& sg = h->h_tab[k];
+      const BatchImage& im = h->img[k];
+      sg = BatchSeg{};
+      sg.W = im.W; sg.H = im.H; sg.rr = im.rr;
+      sg.row0 = row; sg.col0 = blk; sg.tile0 = tile;
+      sg.inv_n = (float)(1.0 / ((double)im.W * im.H));
+      row += im.H;
+      blk += (p.nwx * im.W + ipb - 1) / ipb;
+      tile += dpra_ls_tiles(im.W, im.H);
+    }
+    BatchClass& b = c.bc;
+    b.tab = h->d_tab + c.b;
+    b.nseg = c.e - c.b;
+    b.nrows = row;
+    b.ncolblk = blk;
+    b.planx = p.planx; b.plany = p.plany;
+    b.cols_waves = p.cols_waves; b.rows_waves = p.rows_waves;
+    b.nwx = p.nwx; b.nwy = p.nwy; b.scale = p.scale;
+    b.Hx = p.d_Hx; b.Hy = p.d_Hy; b.twx = p.d_twx; b.twy = p.d_twy;
+    b.grid_rows = batch_grid_rows(p.planx, p.rows_waves, row);
+    b.grid_cols = batch_grid_cols(p.plany, p.cols_waves, blk);
+    if (b.grid_rows < 1 || b.grid_cols < 1)
+      return bail(fail(GLSU_ERR_DEVICE, "occupancy query failed"));
+  }
+  h->ntiles = tile;
+  h->grid_ls = batch_grid_ls(tile);
+  if (h->grid_ls < 1) return bail(fail(GLSU_ERR_DEVICE, "occupancy query failed"));
+  hipEventCreate(&h->e0);
+  hipEventCreate(&h->e1);
+  *out = h;
+  return GLSU_OK;
+}
+
+// one frame of every sequence; the device frame counter selects frames,
+// outputs and the reference/current buffers
+static int batch_step(glsu_dpra_batch* h, int dtype, hipStream_t st) {
+  HIP_TRY(launch_dpra_ls_batch(h->d_tab, (int)h->img.size(), h->ntiles, h->grid_ls, dtype,
+                               h->d_frame, st));
+  HIP_TRY(launch_thr_batch(h->d_tab, (int)h->img.size(), st));
+  for (auto& c : h->cls) HIP_TRY(launch_rows_fwd_batch(c.bc, st));
+  for (auto& c : h->cls) HIP_TRY(launch_cols_batch(c.bc, st));
+  for (auto& c : h->cls) HIP_TRY(launch_rows_inv_batch(c.bc, h->d_frame, st));
+  HIP_TRY(launch_frame_advance(h->d_frame, st));
+  return GLSU_OK;
+}
+
+static int swaps_before(int t, int rr) { return t == 0 ? 0 : (t - 1) / rr + 1; }
+
+extern "C" int glsu_dpra_batch_run(glsu_dpra_batch* h, const void* const* frames, int nframes,
+                                   int dtype, float* const* dphi_all, int use_graph,
+                                   double* ms) {
+  if (!h || !frames || nframes < 0) return fail(GLSU_ERR_ARG, "invalid argument");
+  if (!elem_size(dtype)) return fail(GLSU_ERR_ARG, "unknown frame dtype %d", dtype);
+  for (int i = 0; i < h->n; ++i)
+    if (!frames[i]) return fail(GLSU_ERR_ARG, "sequence %d: null frames", i);
+  if (nframes == 0) return GLSU_OK;
+  HIP_TRY(hipStreamSynchronize(h->s));  // staging buffers are free
+  for (size_t k = 0; k < h->img.size(); ++k) {
+    const BatchImage& im = h->img[k];
+    BatchSeg& sg = h->h_tab[k];
+    sg.frames = frames[im.seq];
+    sg.dphi_out = dphi_all ? dphi_all[im.seq] : nullptr;
+    for (int b = 0; b < 2; ++b) {
+      sg.phi[b] = im.phi[b]; sg.dhi[b] = im.dhi[b]; sg.dlo[b] = im.dlo[b];
+    }
+    sg.z = im.z; sg.A = im.A; sg.D = im.D; sg.plane = im.plane;
+    sg.thr = im.thr; sg.red = im.red;
+  }
+  *h->h_frame = 0;
+  HIP_TRY(hipMemcpyAsync(h->d_tab, h->h_tab, sizeof(BatchSeg) * h->img.size(),
+                         hipMemcpyHostToDevice, h->s));
+  HIP_TRY(hipMemcpyAsync(h->d_frame, h->h_frame, sizeof(int), hipMemcpyHostToDevice, h->s));
+  if (ms) HIP_TRY(hipEventRecord(h->e0, h->s));
+  if (use_graph) {
+    auto it = h->graphs.find(dtype);
+    if (it == h->graphs.end()) {
+      if (!h->cap) HIP_TRY(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
+      HIP_TRY(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
+      int rc = batch_step(h, dtype, h->cap);
+      hipGraph_t g = nullptr;
+      hipError_t ee = hipStreamEndCapture(h->cap, &g);
+      if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+      }
+      if (ee != hipSuccess)
+        return fail(GLSU_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(ee));
+      hipGraphExec_t x = nullptr;
+      HIP_TRY(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+      it = h->graphs.emplace(dtype, std::make_pair(g, x)).first;
+    }
+    for (int t = 0; t < nframes; ++t) HIP_TRY(hipGraphLaunch(it->second.second, h->s));
+  } else {
+    for (int t = 0; t < nframes; ++t)
+      if (int rc = batch_step(h, dtype, h->s)) return rc;
+  }
+  if (ms) {
+    HIP_TRY(hipEventRecord(h->e1, h->s));
+    HIP_TRY(hipEventSynchronize(h->e1));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, h->e0, h->e1));
+    *ms = t;
+  }
+  // the next run starts from index 0 = reference again
+  for (auto& im : h->img)
+    if (swaps_before(nframes, im.rr) & 1) {
+      std::swap(im.phi[0], im.phi[1]);
+      std::swap(im.dhi[0], im.dhi[1]);
+      std::swap(im.dlo[0], im.dlo[1]);
+    }
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_batch_info(glsu_dpra_batch* h, int* nclasses, int* ntiles) {
+  if (!h) return fail(GLSU_ERR_ARG, "null handle");
+  if (nclasses) *nclasses = (int)h->cls.size();
+  if (ntiles) *ntiles = h->ntiles;
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_batch_get_tuning(glsu_dpra_batch* h, int seq, glsu_tuning* t) {
+  if (!h || !t || seq < 0 || seq >= h->n) return fail(GLSU_ERR_ARG, "invalid argument");
+  const int k = h->pos[seq];
+  for (auto& c : h->cls)
+    if (k >= c.b && k < c.e) get_tuning(c.p, t);
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_batch_get_state(glsu_dpra_batch* h, int seq, float* phiref,
+                                         float* dphiref) {
+  if (!h || seq < 0 || seq >= h->n) return fail(GLSU_ERR_ARG, "invalid argument");
+  const BatchImage& im = h->img[h->pos[seq]];
+  const size_t n = (size_t)im.W * im.H;
+  HIP_TRY(hipStreamSynchronize(h->s));
+  if (phiref) HIP_TRY(hipMemcpy(phiref, im.phi[0], n * sizeof(float), hipMemcpyDeviceToHost));
+  if (dphiref) {
+    std::vector<float> hi(n), lo(n);
+    HIP_TRY(hipMemcpy(hi.data(), im.dhi[0], n * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(lo.data(), im.dlo[0], n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) dphiref[i] = hi[i] + lo[i];
+  }
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_batch_destroy(glsu_dpra_batch* h) {
+  if (!h) return GLSU_OK;
+  batch_free(h);
+  delete h;
+  return GLSU_OK;
+}

@@ -25,6 +25,8 @@
 // and z = 1 (the oracle's defined behaviour for that case).
 #include "glsu_internal.h"
 
+#include <algorithm>
+
 #include <cstdint>
 
 namespace glsu {
@@ -73,18 +75,16 @@ __device__ __forceinline__ void sincos_half(float x, float* s, float* c) {
 // fetched from HBM once per L2.
 constexpr int DTX = 64, DTY = 8;
 
+// One 64 x 8 tile (bx, by) of one image; all threads of the block take part.
 template <typename InT>
-__global__ __launch_bounds__(DTX* DTY) void dpra_ls_kernel(const InT* __restrict__ I,
-                                                           const float* __restrict__ phr,
-                                                           float2* __restrict__ z, int W, int H,
-                                                           int pitch) {
+__device__ __forceinline__ void dpra_ls_body(const InT* __restrict__ I,
+                                             const float* __restrict__ phr,
+                                             float2* __restrict__ z, int W, int H, int pitch,
+                                             int bx, int by) {
   __shared__ float sI[DTY + 2][DTX + 2];
   __shared__ float sS[DTY + 2][DTX + 2];  // sin(phi/2)
   __shared__ float sC[DTY + 2][DTX + 2];  // cos(phi/2)
   const int tx = threadIdx.x % DTX, ty = threadIdx.x / DTX;
-  const int nbx = (W + DTX - 1) / DTX;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const int bx = tile % nbx, by = tile / nbx;
   const int x0 = bx * DTX - 1, y0 = by * DTY - 1;
   for (int i = threadIdx.x; i < (DTX + 2) * (DTY + 2); i += DTX * DTY) {
     const int ly = i / (DTX + 2), lx = i % (DTX + 2);
@@ -187,6 +187,76 @@ __global__ __launch_bounds__(DTX* DTY) void dpra_ls_kernel(const InT* __restrict
   }
   z[(size_t)y * W + x] = out;
 }
+
+template <typename InT>
+__global__ __launch_bounds__(DTX* DTY) void dpra_ls_kernel(const InT* __restrict__ I,
+                                                           const float* __restrict__ phr,
+                                                           float2* __restrict__ z, int W, int H,
+                                                           int pitch) {
+  const int nbx = (W + DTX - 1) / DTX;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  dpra_ls_body<InT>(I, phr, z, W, H, pitch, tile % nbx, tile / nbx);
+}
+
+// Batched form: persistent walk over the 64 x 8 tiles of every image in the
+// table; frame t of image s is frames_s + t*W*H, its reference phase the
+// buffer that the swap parity of t names.
+template <typename InT>
+__global__ __launch_bounds__(DTX* DTY) void dpra_ls_batch_kernel(const BatchSeg* __restrict__ tab,
+                                                                 int nseg, int ntiles,
+                                                                 const int* __restrict__ frame) {
+  const int t = *frame;
+  for (int v = blockIdx.x; v < ntiles; v += gridDim.x) {
+    const int tile = xcd_remap(v, ntiles);
+    int lo = 0, hi = nseg - 1;  // last image whose first tile is <= tile
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tab[mid].tile0 <= tile) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchSeg& s = tab[lo];
+    const int nbx = (s.W + DTX - 1) / DTX;
+    const int lt = tile - s.tile0;
+    const int c = (t == 0 ? 0 : (t - 1) / s.rr + 1) & 1;
+    __syncthreads();  // LDS of the previous tile is free
+    dpra_ls_body<InT>((const InT*)s.frames + (size_t)t * s.W * s.H, s.phi[c], s.z, s.W, s.H, s.W,
+                      lt % nbx, lt / nbx);
+  }
+}
+
+hipError_t launch_dpra_ls_batch(const BatchSeg* tab, int nseg, int ntiles, int grid, int dtype,
+                                const int* frame, hipStream_t s) {
+  switch (dtype) {
+    case IN_U8:
+      hipLaunchKernelGGL(dpra_ls_batch_kernel<uint8_t>, dim3(grid), dim3(DTX * DTY), 0, s, tab,
+                         nseg, ntiles, frame);
+      break;
+    case IN_F32:
+      hipLaunchKernelGGL(dpra_ls_batch_kernel<float>, dim3(grid), dim3(DTX * DTY), 0, s, tab,
+                         nseg, ntiles, frame);
+      break;
+    case IN_F64:
+      hipLaunchKernelGGL(dpra_ls_batch_kernel<double>, dim3(grid), dim3(DTX * DTY), 0, s, tab,
+                         nseg, ntiles, frame);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+int batch_grid_ls(int ntiles) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, dpra_ls_batch_kernel<float>, DTX * DTY,
+                                                    0) != hipSuccess)
+    return -1;
+  const int cap = std::max(8, (cus * std::max(per, 1)) / 8 * 8);
+  return ntiles <= cap ? ntiles : cap;
+}
+
+int dpra_ls_tiles(int W, int H) { return ((W + DTX - 1) / DTX) * ((H + DTY - 1) / DTY); }
 
 hipError_t launch_dpra_ls(const void* frame, int dtype, const float* phiref, float2* z, int W,
                           int H, int pitch, hipStream_t s) {

@@ -56,7 +56,8 @@ constexpr int kRowWavesMax = 4;  // row kernels: one wavefront per transform, <=
 
 int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh, double wxi,
                   double sigy, double wyl, double wyh, double wyi, double thr, int planx,
-                  int plany, int cols_waves, int rows_waves, std::string& err);
+                  int plany, int cols_waves, int rows_waves, std::string& err,
+                  bool buffers = true);  // false: spectra/twiddle tables only
 void wff_plan_free(WffPlan& p);
 
 // Kernel launchers (all asynchronous on `s`).
@@ -77,6 +78,57 @@ struct UpdateArgs {
   float2* filtered;  // may be null
 };
 hipError_t launch_rows_inv_update(const WffPlan& p, const UpdateArgs& u, hipStream_t s);
+
+// ---- batched launches over many images --------------------------------
+// One image of a batched launch (device-resident table entry).  row0 / col0 /
+// tile0 place the image in the item space of its plan class's row launches,
+// column launches (in column-kernel blocks) and of the least-squares launch.
+// phi / dhi / dlo are double buffers whose reference/current roles follow the
+// frame counter and rr (see swap_parity).
+struct BatchSeg {
+  int W, H, rr;
+  int row0, col0, tile0;
+  float inv_n;               // 1 / (W*H), for the threshold
+  int pad;
+  const void* frames;        // [T][H][W] frames of this image
+  float* dphi_out;           // [T][H][W] cumulative phase out (may be null)
+  float* phi[2];
+  float* dhi[2];
+  float* dlo[2];
+  float2* z;
+  float2* A;
+  float2* D;
+  size_t plane;
+  float* thr;                // < 0 until computed from the first frame
+  float* red;                // 256 partial sums
+};
+
+// Plan-class launch data of the batched WFF2 stages.
+struct BatchClass {
+  const BatchSeg* tab;       // device table of this class's images (contiguous)
+  int nseg;
+  int nrows;                 // sum of H
+  int ncolblk;               // sum over images of ceil(nwx*W / items per block)
+  int planx, plany, cols_waves, rows_waves;
+  int nwx, nwy;
+  float scale;
+  const float2 *Hx, *Hy, *twx, *twy;
+  int grid_rows, grid_cols;  // persistent grid sizes (multiples of 8)
+};
+hipError_t launch_rows_fwd_batch(const BatchClass& c, hipStream_t s);
+hipError_t launch_cols_batch(const BatchClass& c, hipStream_t s);
+hipError_t launch_rows_inv_batch(const BatchClass& c, const int* frame, hipStream_t s);
+// items per column-kernel block for a plan and wave count
+int cols_items_per_block(int plany, int waves);
+// occupancy-limited resident workgroups of each batched kernel on the device
+int batch_grid_rows(int planx, int rows_waves, int nrows);
+int batch_grid_cols(int plany, int cols_waves, int nblk);
+hipError_t launch_dpra_ls_batch(const BatchSeg* tab, int nseg, int ntiles, int grid, int dtype,
+                                const int* frame, hipStream_t s);
+int dpra_ls_tiles(int W, int H);
+int batch_grid_ls(int ntiles);
+hipError_t launch_thr_batch(const BatchSeg* tab, int nseg, hipStream_t s);
+hipError_t launch_frame_advance(int* frame, hipStream_t s);
 
 // DPRA least squares: z = exp(j*Delta) from frame + phiref
 enum InDtype { IN_U8 = 0, IN_F32 = 1, IN_F64 = 2 };

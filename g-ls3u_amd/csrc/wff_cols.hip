@@ -38,4 +38,68 @@ hipError_t launch_cols(const WffPlan& p, hipStream_t s) {
   return hipGetLastError();
 }
 
+int cols_items_per_block(int plany, int waves) {
+  for (auto& q : fft_plans())
+    if (q.id == plany) return waves * 64 / q.T;
+  return 0;
+}
+
+template <class K>
+static int persistent_grid(K kernel, int block, int smem, int items) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, smem) != hipSuccess)
+    return -1;
+  const int cap = std::max(8, (cus * std::max(per, 1)) / 8 * 8);
+  return items <= cap ? items : cap;
+}
+
+int batch_grid_cols(int plany, int waves, int nblk) {
+  const int smem = plan_smem_bytes(plany, waves, true);
+  switch (plany) {
+#define BODY                                                                              \
+  switch (waves) {                                                                        \
+    case 1:                                                                               \
+      return persistent_grid(wff_cols_batch_kernel<P, 1>, 64, smem, nblk);                \
+    case 2:                                                                               \
+      return persistent_grid(wff_cols_batch_kernel<P, 2>, 128, smem, nblk);               \
+    default:                                                                              \
+      return persistent_grid(wff_cols_batch_kernel<P, 4>, 256, smem, nblk);               \
+  }
+    GLSU_PLANS(GLSU_DISPATCH_CASE)
+#undef BODY
+    default:
+      return -1;
+  }
+  return -1;
+}
+
+hipError_t launch_cols_batch(const BatchClass& c, hipStream_t s) {
+  const int waves = c.cols_waves;
+  const int smem = plan_smem_bytes(c.plany, waves, true);
+  switch (c.plany) {
+#define BODY                                                                                 \
+  switch (waves) {                                                                           \
+    case 1:                                                                                  \
+      hipLaunchKernelGGL((wff_cols_batch_kernel<P, 1>), dim3(c.grid_cols), dim3(64), smem, s, \
+                         c.tab, c.nseg, c.ncolblk, c.Hy, c.twy, c.nwx, c.nwy);               \
+      break;                                                                                 \
+    case 2:                                                                                  \
+      hipLaunchKernelGGL((wff_cols_batch_kernel<P, 2>), dim3(c.grid_cols), dim3(128), smem,  \
+                         s, c.tab, c.nseg, c.ncolblk, c.Hy, c.twy, c.nwx, c.nwy);            \
+      break;                                                                                 \
+    default:                                                                                 \
+      hipLaunchKernelGGL((wff_cols_batch_kernel<P, 4>), dim3(c.grid_cols), dim3(256), smem,  \
+                         s, c.tab, c.nseg, c.ncolblk, c.Hy, c.twy, c.nwx, c.nwy);            \
+      break;                                                                                 \
+  }
+    GLSU_PLANS(GLSU_DISPATCH_CASE)
+#undef BODY
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 }  // namespace glsu

@@ -51,17 +51,17 @@ __device__ __forceinline__ size_t tile_index(int y, int x, int H) {
 // Wave 0 computes Zr = FFT_x(f row) and shares it through LDS; wave w then
 // produces A_wx = IFFT_x(Zr * Hx_wx) for wx = w, w + RW, ...
 template <class P>
-__global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_kernel(
-    const float2* __restrict__ f, float2* __restrict__ A, const float2* __restrict__ Hx,
-    const float2* __restrict__ tw_g, int W, int H, int nwx, size_t plane) {
+__device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ f,
+                                              float2* __restrict__ A,
+                                              const float2* __restrict__ Hx,
+                                              const float2* __restrict__ tw_g, int W, int H,
+                                              int nwx, size_t plane, int y, int lane,
+                                              float2* smem) {
   static_assert(P::T == 64, "row kernels use one wavefront per transform");
   constexpr int T = P::T, E = P::E, N = P::N;
-  extern __shared__ float2 smem[];
   const int RW = blockDim.x / T;
-  const int wave = threadIdx.x / T, lane = threadIdx.x % T;
+  const int wave = threadIdx.x / T;
   float2* buf = smem + wave * P::LDS;
-  const int y = xcd_remap(blockIdx.x, gridDim.x);  // uniform per workgroup
-  if (y >= H) return;
   FftLane<P> ctx;
   ctx.init(lane, tw_g);
   const int kb = P::kbase(lane);
@@ -99,27 +99,30 @@ __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_kernel(
   }
 }
 
+template <class P>
+__global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_kernel(
+    const float2* __restrict__ f, float2* __restrict__ A, const float2* __restrict__ Hx,
+    const float2* __restrict__ tw_g, int W, int H, int nwx, size_t plane) {
+  extern __shared__ float2 smem[];
+  const int y = xcd_remap(blockIdx.x, gridDim.x);  // uniform per workgroup
+  if (y >= H) return;
+  rows_fwd_body<P>(f, A, Hx, tw_g, W, H, nwx, plane, y, threadIdx.x % P::T, smem);
+}
+
 // -------------------------------------------------------------- columns
 // One transform slot per (wx, column).  Ahat = FFT_y(A col); for each wy:
 // S = IFFT_y(Ahat*Hy); T = threshold(S) on image rows; acc += FFT_y(T)*Hy.
-// D col = IFFT_y(acc).
-template <class P, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
-    const float2* __restrict__ A, float2* __restrict__ D, const float2* __restrict__ Hy,
-    const float2* __restrict__ tw_g, const float* __restrict__ thr_p, int W, int H, int nwx,
-    int nwy, size_t plane) {
+// D col = IFFT_y(acc).  `ttab` (transition twiddles in LDS) is filled by the
+// caller before the first item.
+template <class P>
+__device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* __restrict__ D,
+                                          const float2* __restrict__ Hy,
+                                          const float2* __restrict__ tw_g, float thr, int W,
+                                          int H, int nwx, int nwy, size_t plane, int item,
+                                          int lane, const float2* ttab, float2* buf) {
   constexpr int T = P::T, E = P::E, N = P::N;
-  extern __shared__ float2 smem[];
   using Ctx = FftLane<P, true>;
-  float2* ttab = smem;
-  Ctx::fill_ttab(ttab, tw_g);
-  __syncthreads();
-  const int slot = threadIdx.x / T, lane = threadIdx.x % T;
-  float2* buf = smem + E * T + slot * P::LDS;
-  const int item = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / T) + slot;
-  if (item >= nwx * W) return;
   const int k = item / W, x = item % W;
-  const float thr = *thr_p;
   const float thr2 = thr * thr;
   Ctx ctx;
   ctx.init(lane, tw_g, ttab);
@@ -169,6 +172,23 @@ __global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
   }
 }
 
+template <class P, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
+    const float2* __restrict__ A, float2* __restrict__ D, const float2* __restrict__ Hy,
+    const float2* __restrict__ tw_g, const float* __restrict__ thr_p, int W, int H, int nwx,
+    int nwy, size_t plane) {
+  constexpr int T = P::T, E = P::E;
+  extern __shared__ float2 smem[];
+  float2* ttab = smem;
+  FftLane<P, true>::fill_ttab(ttab, tw_g);
+  __syncthreads();
+  const int slot = threadIdx.x / T;
+  float2* buf = smem + E * T + slot * P::LDS;
+  const int item = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / T) + slot;
+  if (item >= nwx * W) return;
+  cols_body<P>(A, D, Hy, tw_g, *thr_p, W, H, nwx, nwy, plane, item, threadIdx.x % T, ttab, buf);
+}
+
 // ---------------------------------------------------------- row inverse
 // One workgroup per row, RW waves.  Wave w accumulates FFT_x(D_wx row) * Hx_wx
 // over wx = w, w + RW, ...; the partial spectra are summed through LDS and
@@ -176,18 +196,17 @@ __global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
 // MODE 0: write F.  MODE 1: DPRA phase update (DPRA/dpra_cpu.cpp:244-251):
 //   t = angle(F); phicurr = wrap(phiref + t); dphi = dphiref + t (two-float sum).
 template <class P, int MODE>
-__global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_inv_kernel(
-    const float2* __restrict__ D, const float2* __restrict__ Hx, const float2* __restrict__ tw_g,
-    int W, int H, int nwx, size_t plane, float scale, float2* __restrict__ filtered,
-    UpdateArgs u) {
+__device__ __forceinline__ void rows_inv_body(const float2* __restrict__ D,
+                                              const float2* __restrict__ Hx,
+                                              const float2* __restrict__ tw_g, int W, int H,
+                                              int nwx, size_t plane, float scale,
+                                              float2* __restrict__ filtered, const UpdateArgs& u,
+                                              int y, int lane, float2* smem) {
   static_assert(P::T == 64, "row kernels use one wavefront per transform");
   constexpr int T = P::T, E = P::E, N = P::N;
-  extern __shared__ float2 smem[];
   const int RW = blockDim.x / T;
-  const int wave = threadIdx.x / T, lane = threadIdx.x % T;
+  const int wave = threadIdx.x / T;
   float2* buf = smem + wave * P::LDS;
-  const int y = xcd_remap(blockIdx.x, gridDim.x);
-  if (y >= H) return;
   FftLane<P> ctx;
   ctx.init(lane, tw_g);
   const int kb = P::kbase(lane);
@@ -257,6 +276,109 @@ __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_inv_kernel(
       u.dphi_lo[id] = nl;
       if (u.dphi_out) u.dphi_out[id] = nh;
     }
+  }
+}
+
+template <class P, int MODE>
+__global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_inv_kernel(
+    const float2* __restrict__ D, const float2* __restrict__ Hx, const float2* __restrict__ tw_g,
+    int W, int H, int nwx, size_t plane, float scale, float2* __restrict__ filtered,
+    UpdateArgs u) {
+  extern __shared__ float2 smem[];
+  const int y = xcd_remap(blockIdx.x, gridDim.x);
+  if (y >= H) return;
+  rows_inv_body<P, MODE>(D, Hx, tw_g, W, H, nwx, plane, scale, filtered, u, y,
+                         threadIdx.x % P::T, smem);
+}
+
+// ------------------------------------------------- batched (many images)
+// Persistent kernels over a device table of images (BatchSeg) that share one
+// FFT plan pair: each workgroup walks virtual work items v = blockIdx.x,
+// blockIdx.x + gridDim.x, ... (XCD-contiguous order), finds the image that
+// owns the item by binary search on the table's offsets, and runs the same
+// per-item body as the single-image kernels, so every image's result is
+// bit-identical to a single-image run with the same plan and launch shape.
+
+// The lane index, recomputed behind an empty asm for every item: per-lane
+// twiddles and addresses derived from it are then rebuilt per item instead of
+// being hoisted out of the persistent loop, where they would stay live in
+// VGPRs across the whole body and halve occupancy.
+template <class P>
+__device__ __forceinline__ int opaque_lane() {
+  int lane = threadIdx.x % P::T;
+  __asm__ volatile("" : "+v"(lane));
+  return lane;
+}
+
+// index of the last table entry whose offset (field `F`) is <= key
+template <int BatchSeg::*F>
+__device__ __forceinline__ int seg_find(const BatchSeg* __restrict__ tab, int n, int key) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].*F <= key) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class P>
+__global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_batch_kernel(
+    const BatchSeg* __restrict__ tab, int nseg, int nrows, const float2* __restrict__ Hx,
+    const float2* __restrict__ tw_g, int nwx) {
+  extern __shared__ float2 smem[];
+  for (int v = blockIdx.x; v < nrows; v += gridDim.x) {
+    const int r = xcd_remap(v, nrows);
+    const BatchSeg& s = tab[seg_find<&BatchSeg::row0>(tab, nseg, r)];
+    __syncthreads();  // LDS of the previous row is free
+    rows_fwd_body<P>(s.z, s.A, Hx, tw_g, s.W, s.H, nwx, s.plane, r - s.row0, opaque_lane<P>(),
+                     smem);
+  }
+}
+
+template <class P, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void wff_cols_batch_kernel(
+    const BatchSeg* __restrict__ tab, int nseg, int nblk, const float2* __restrict__ Hy,
+    const float2* __restrict__ tw_g, int nwx, int nwy) {
+  constexpr int T = P::T, E = P::E;
+  extern __shared__ float2 smem[];
+  float2* ttab = smem;
+  FftLane<P, true>::fill_ttab(ttab, tw_g);
+  __syncthreads();
+  const int slot = threadIdx.x / T;
+  float2* buf = smem + E * T + slot * P::LDS;
+  for (int v = blockIdx.x; v < nblk; v += gridDim.x) {
+    const int b = xcd_remap(v, nblk);
+    const BatchSeg& s = tab[seg_find<&BatchSeg::col0>(tab, nseg, b)];
+    const int item = (b - s.col0) * (blockDim.x / T) + slot;
+    if (item < nwx * s.W)
+      cols_body<P>(s.A, s.D, Hy, tw_g, *s.thr, s.W, s.H, nwx, nwy, s.plane, item,
+                   opaque_lane<P>(), ttab, buf);
+  }
+}
+
+// number of reference-phase swaps before frame t (swap after frame i when
+// i % rr == 0, DPRA/dpra_cpu.cpp:84-87); its parity says which buffer of each
+// double buffer holds the reference
+__device__ __forceinline__ int swap_parity(int t, int rr) {
+  return (t == 0 ? 0 : (t - 1) / rr + 1) & 1;
+}
+
+template <class P>
+__global__ __launch_bounds__(64 * kRowWavesMax, 2) void wff_rows_inv_batch_kernel(
+    const BatchSeg* __restrict__ tab, int nseg, int nrows, const float2* __restrict__ Hx,
+    const float2* __restrict__ tw_g, int nwx, float scale, const int* __restrict__ frame) {
+  extern __shared__ float2 smem[];
+  const int t = *frame;
+  for (int v = blockIdx.x; v < nrows; v += gridDim.x) {
+    const int r = xcd_remap(v, nrows);
+    const BatchSeg& s = tab[seg_find<&BatchSeg::row0>(tab, nseg, r)];
+    const int c = swap_parity(t, s.rr);
+    UpdateArgs u{s.phi[c], s.dhi[c], s.dlo[c], s.phi[c ^ 1], s.dhi[c ^ 1], s.dlo[c ^ 1],
+                 s.dphi_out ? s.dphi_out + (size_t)t * s.W * s.H : nullptr, nullptr};
+    __syncthreads();
+    rows_inv_body<P, 1>(s.D, Hx, tw_g, s.W, s.H, nwx, s.plane, scale, nullptr, u, r - s.row0,
+                        opaque_lane<P>(), smem);
   }
 }
 

@@ -62,6 +62,60 @@ hipError_t launch_thr(const WffPlan& p, const float2* f, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Batched threshold: blocks [256 s, 256 s + 256) reduce image s exactly as
+// thr_partial_kernel does with a 256-block grid (same strides, same tree), so
+// each image's threshold is bit-identical to a single-image run.  Images whose
+// threshold is already set (>= 0) are skipped.
+__global__ __launch_bounds__(256) void thr_partial_batch_kernel(const BatchSeg* __restrict__ tab) {
+  const BatchSeg& sg = tab[blockIdx.x / 256];
+  if (*sg.thr >= 0.f) return;
+  const int b = blockIdx.x % 256;
+  const size_t n = (size_t)sg.W * sg.H;
+  float s = 0.f;
+  for (size_t i = (size_t)b * blockDim.x + threadIdx.x; i < n; i += (size_t)256 * blockDim.x) {
+    const float2 v = sg.z[i];
+    s += v.x * v.x + v.y * v.y;
+  }
+  __shared__ float red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sg.red[b] = red[0];
+}
+
+__global__ __launch_bounds__(256) void thr_final_batch_kernel(const BatchSeg* __restrict__ tab) {
+  const BatchSeg& sg = tab[blockIdx.x];
+  if (*sg.thr >= 0.f) return;
+  __shared__ float red[256];
+  float s = 0.f;
+  for (int i = threadIdx.x; i < 256; i += 256) s += sg.red[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *sg.thr = 6.f * sqrtf(red[0] * sg.inv_n / 3.f);
+}
+
+hipError_t launch_thr_batch(const BatchSeg* tab, int nseg, hipStream_t s) {
+  hipLaunchKernelGGL(thr_partial_batch_kernel, dim3(256 * nseg), dim3(256), 0, s, tab);
+  hipLaunchKernelGGL(thr_final_batch_kernel, dim3(nseg), dim3(256), 0, s, tab);
+  return hipGetLastError();
+}
+
+__global__ void frame_advance_kernel(int* frame) {
+  if (threadIdx.x == 0) *frame += 1;
+}
+
+hipError_t launch_frame_advance(int* frame, hipStream_t s) {
+  hipLaunchKernelGGL(frame_advance_kernel, dim3(1), dim3(64), 0, s, frame);
+  return hipGetLastError();
+}
+
 // --------------------------------------------------------- plan set-up
 static const FftPlanInfo* find_plan(int id) {
   for (auto& p : fft_plans())
@@ -120,7 +174,8 @@ static hipError_t upload(V*& dptr, const std::vector<V>& h) {
 
 int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh, double wxi,
                   double sigy, double wyl, double wyh, double wyi, double thr, int planx,
-                  int plany, int cols_waves, int rows_waves, std::string& err) {
+                  int plany, int cols_waves, int rows_waves, std::string& err,
+                  bool buffers) {
   p = WffPlan{};
   p.W = W;
   p.H = H;
@@ -180,8 +235,13 @@ int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh,
   hipError_t e;
   if ((e = upload(p.d_Hx, hx)) != hipSuccess || (e = upload(p.d_Hy, hy)) != hipSuccess ||
       (e = upload(p.d_twx, twiddles(p.Nx))) != hipSuccess ||
-      (e = upload(p.d_twy, twiddles(p.Ny))) != hipSuccess ||
-      (e = hipMalloc((void**)&p.d_A, sizeof(float2) * p.plane_elems * p.nwx)) != hipSuccess ||
+      (e = upload(p.d_twy, twiddles(p.Ny))) != hipSuccess) {
+    err = std::string("device allocation failed: ") + hipGetErrorString(e);
+    wff_plan_free(p);
+    return -4;
+  }
+  if (!buffers) return 0;
+  if ((e = hipMalloc((void**)&p.d_A, sizeof(float2) * p.plane_elems * p.nwx)) != hipSuccess ||
       (e = hipMalloc((void**)&p.d_D, sizeof(float2) * p.plane_elems * p.nwx)) != hipSuccess ||
       (e = hipMalloc((void**)&p.d_thr, sizeof(float))) != hipSuccess ||
       (e = hipMalloc((void**)&p.d_red, sizeof(float) * 256)) != hipSuccess) {

@@ -34,4 +34,23 @@ hipError_t launch_rows_inv_update(const WffPlan& p, const UpdateArgs& u, hipStre
   return launch_rows_inv_impl<1>(p, nullptr, u, s);
 }
 
+hipError_t launch_rows_inv_batch(const BatchClass& c, const int* frame, hipStream_t s) {
+  const int rw = c.rows_waves;
+  const int smem = plan_smem_bytes(c.planx, rw);
+  switch (c.planx) {
+#define BODY                                                                                  \
+  if constexpr (P::T == 64) {                                                                 \
+    hipLaunchKernelGGL((wff_rows_inv_batch_kernel<P>), dim3(c.grid_rows), dim3(64 * rw), smem, \
+                       s, c.tab, c.nseg, c.nrows, c.Hx, c.twx, c.nwx, c.scale, frame);        \
+  } else {                                                                                    \
+    return hipErrorInvalidValue;                                                              \
+  }
+    GLSU_PLANS(GLSU_DISPATCH_CASE)
+#undef BODY
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 }  // namespace glsu

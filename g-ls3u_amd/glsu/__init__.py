@@ -10,4 +10,5 @@ from ._lib import (GlsuError, PaddingError, WffParams, Tuning, build, check, lib
 from .params import wff_params, dpra_reference_params, wft2_default_params  # noqa: F401
 from .wft2 import WFT2  # noqa: F401
 from .dpra import DPRA, gen_frames  # noqa: F401
+from .batch import DPRABatch  # noqa: F401
 from . import synth, autotune  # noqa: F401

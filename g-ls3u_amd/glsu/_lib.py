@@ -46,6 +46,12 @@ class Tuning(ctypes.Structure):
         return Tuning(int(planx), int(plany), int(cols_waves), int(rows_waves))
 
 
+class SeqDesc(ctypes.Structure):
+    """glsu_seq_desc: one sequence of a batch."""
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("rr", ctypes.c_int),
+                ("phi0", ctypes.POINTER(ctypes.c_double))]
+
+
 def tuning_arg(tuning=None, planx=-1, plany=-1):
     """Tuning struct from a Tuning, a dict with any of its fields, or None
     (planx / plany given separately override the dict)."""
@@ -122,6 +128,13 @@ def lib():
         "glsu_dpra_thr": (I, [V, PD]),
         "glsu_dpra_destroy": (I, [V]),
         "glsu_gen_frames": (I, [V, I, I, I, I, U64, D, D, D, D, V]),
+        "glsu_dpra_batch_create": (I, [PV, ctypes.POINTER(SeqDesc), I, PP, V, PT]),
+        "glsu_dpra_batch_run": (I, [V, ctypes.POINTER(ctypes.c_void_p), I, I,
+                                    ctypes.POINTER(ctypes.c_void_p), I, PD]),
+        "glsu_dpra_batch_info": (I, [V, PI, PI]),
+        "glsu_dpra_batch_get_tuning": (I, [V, I, PT]),
+        "glsu_dpra_batch_get_state": (I, [V, I, V, V]),
+        "glsu_dpra_batch_destroy": (I, [V]),
         "glsu_fft_plans": (I, [PI, PI, PI, PI, I, PI]),
     }
     for name, (res, args) in sig.items():
@@ -152,7 +165,9 @@ def exported_symbols():
         "glsu_dpra_frame",
         "glsu_dpra_frame_host", "glsu_dpra_update_ref", "glsu_dpra_run", "glsu_dpra_run_graph",
         "glsu_dpra_profile", "glsu_dpra_time_kernel", "glsu_dpra_get_state", "glsu_dpra_set_state", "glsu_dpra_copy_z", "glsu_dpra_thr",
-        "glsu_dpra_destroy", "glsu_gen_frames", "glsu_fft_plans",
+        "glsu_dpra_destroy", "glsu_gen_frames", "glsu_dpra_batch_create", "glsu_dpra_batch_run",
+        "glsu_dpra_batch_info", "glsu_dpra_batch_get_tuning", "glsu_dpra_batch_get_state",
+        "glsu_dpra_batch_destroy", "glsu_fft_plans",
     ]
 
 

@@ -172,6 +172,37 @@ int glsu_dpra_thr(glsu_dpra *h, double *thr);
 int glsu_dpra_get_tuning(glsu_dpra *h, glsu_tuning *t);
 int glsu_dpra_destroy(glsu_dpra *h);
 
+/* ---- many sequences at once ----------------------------------------------
+ * A batch runs the DPRA chain of n independent sequences of any sizes
+ * together: per frame, one persistent least-squares launch over all images
+ * and, per group of sequences sharing FFT plans and launch shapes, one
+ * persistent launch per WFF2 stage, replayed from one captured hipGraph.
+ * Each sequence behaves as its own DPRA_CPU / DPRA_CUDAF object
+ * (DPRA/dpra_cpu.h:30-48: ctor(Phi0, W, H, rr), operator()(frames, dPhi_Sum,
+ * time), update every frame with i % rr == 0) and gives bit-identical results
+ * to glsu_dpra_run on a handle created with the same tuning.  The reference
+ * has no batched form; its application runs one sequence at a time
+ * (App_DPRA/dprawidget.cpp:223-264). */
+typedef struct glsu_seq_desc {
+  int width, height, rr;
+  const double *phi0;      /* host, height*width initial phase */
+} glsu_seq_desc;
+typedef struct glsu_dpra_batch glsu_dpra_batch;
+/* tunings: n entries (one per sequence) or NULL for the defaults. */
+int glsu_dpra_batch_create(glsu_dpra_batch **out, const glsu_seq_desc *seqs, int n,
+                           const glsu_wff_params *p, void *hip_stream,
+                           const glsu_tuning *tunings);
+/* d_frames[i]: device frames of sequence i (nframes*height*width, frame_dtype);
+ * d_dphi_all: NULL or n device outputs (nframes*height*width float32).
+ * Every call processes frames 0..nframes-1 of every sequence, continuing
+ * from the state the previous call left (as repeated operator() calls). */
+int glsu_dpra_batch_run(glsu_dpra_batch *h, const void *const *d_frames, int nframes,
+                        int frame_dtype, float *const *d_dphi_all, int use_graph, double *ms);
+int glsu_dpra_batch_info(glsu_dpra_batch *h, int *nclasses, int *ntiles);
+int glsu_dpra_batch_get_tuning(glsu_dpra_batch *h, int seq, glsu_tuning *t);
+int glsu_dpra_batch_get_state(glsu_dpra_batch *h, int seq, float *phiref, float *dphiref);
+int glsu_dpra_batch_destroy(glsu_dpra_batch *h);
+
 /* ---- utilities ---------------------------------------------------------- */
 /* Seeded synthetic fringe frames, written to device memory (float32,
  * nframes*height*width): I = 128 + 100 cos(phi) + noise*N(0,1),

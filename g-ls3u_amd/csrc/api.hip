@@ -35,6 +35,11 @@ static int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+namespace glsu {
+void set_error(const char* msg) { g_err = msg; }
+int unet_forward_on(glsu_unet* h, const float2* x, float2* out, int H, int W, hipStream_t s);
+}  // namespace glsu
+
 #define HIP_TRY(expr)                                                                   \
   do {                                                                                  \
     hipError_t e_ = (expr);                                                             \
@@ -366,6 +371,13 @@ struct glsu_dpra {
   float* d_dphi_stage = nullptr;
   hipStream_t cap = nullptr;  // capture stream for glsu_dpra_run_graph
   std::map<GraphKey, GraphEntry> graphs;
+  // learned path (glsu_dpra_frame_learned)
+  float2* d_net = nullptr;                      // network output, H*W
+  float2 *d_zwin = nullptr, *d_fwin = nullptr;  // spot-check window in / filtered
+  size_t win_cap = 0;
+  std::map<std::pair<int, int>, WffPlan> wins;  // WFF2 plans by window size
+  float* d_res = nullptr;
+  float* h_res = nullptr;                       // pinned
 };
 
 static int dpra_alloc(glsu_dpra* h) {
@@ -395,6 +407,9 @@ static void dpra_free(glsu_dpra* h) {
   hipFree(h->phiref); hipFree(h->phicurr);
   hipFree(h->dref_hi); hipFree(h->dref_lo); hipFree(h->dphi_hi); hipFree(h->dphi_lo);
   hipFree(h->z); hipFree(h->d_stage); hipFree(h->d_dphi_stage);
+  hipFree(h->d_net); hipFree(h->d_zwin); hipFree(h->d_fwin); hipFree(h->d_res);
+  if (h->h_res) hipHostFree(h->h_res);
+  for (auto& kv : h->wins) wff_plan_free(kv.second);
   wff_plan_free(h->p);
   if (h->e0) hipEventDestroy(h->e0);
   if (h->e1) hipEventDestroy(h->e1);
@@ -1071,5 +1086,97 @@ extern "C" int glsu_dpra_batch_destroy(glsu_dpra_batch* h) {
   if (!h) return GLSU_OK;
   batch_free(h);
   delete h;
+  return GLSU_OK;
+}
+
+// ============================================================ learned frame
+// One DPRA frame with the U-Net standing in for WFF2 (LS -> z -> net -> phase
+// update), with a WFF2 spot check: the 64 x 64 tile at (tile_x, tile_y) is
+// filtered exactly by WFF2 on its window (tile +- 2s, clipped to the image;
+// each output pixel of WFF2 depends on its input within +-2s, so the window
+// gives the full-image values on the tile), and when the RMS of the wrapped
+// difference between the network's and WFF2's phase on the tile exceeds tol
+// the frame is filtered by WFF2 instead.  tol < 0 skips the check.
+extern "C" int glsu_dpra_frame_learned(glsu_dpra* h, glsu_unet* net, const void* frame,
+                                       int dtype, float* d_dphi, double tol, int tile_x,
+                                       int tile_y, double* residual, int* fallback) {
+  if (!h || !net || !frame) return fail(GLSU_ERR_ARG, "null argument");
+  if (!elem_size(dtype)) return fail(GLSU_ERR_ARG, "unknown frame dtype %d", dtype);
+  const int W = h->W, H = h->H;
+  if (tol >= 0 && (W < 64 || H < 64)) return fail(GLSU_ERR_ARG, "spot check needs >= 64x64");
+  WffPlan& p = h->p;
+  const size_t n = (size_t)W * H;
+  if (!h->d_net) HIP_TRY(hipMalloc((void**)&h->d_net, n * sizeof(float2)));
+  HIP_TRY(launch_dpra_ls(frame, dtype, h->phiref, h->z, W, H, W, h->s));
+  if (!p.thr_ready) {
+    HIP_TRY(launch_thr(p, h->z, h->s));
+    p.thr_ready = true;
+  }
+  if (int rc = unet_forward_on(net, h->z, h->d_net, H, W, h->s)) return rc;
+  double res = -1.0;
+  if (tol >= 0) {
+    const int tx = std::min(std::max(tile_x, 0), W - 64), ty = std::min(std::max(tile_y, 0), H - 64);
+    const int x0 = std::max(tx - 2 * p.sx, 0), x1 = std::min(tx + 64 + 2 * p.sx, W);
+    const int y0 = std::max(ty - 2 * p.sy, 0), y1 = std::min(ty + 64 + 2 * p.sy, H);
+    const int Ww = x1 - x0, Hw = y1 - y0;
+    auto key = std::make_pair(Ww, Hw);
+    auto it = h->wins.find(key);
+    if (it == h->wins.end()) {
+      WffPlan wp;
+      std::string err;
+      int rc = wff_plan_init(wp, Ww, Hw, p.sigx, p.wxl, p.wxh, p.wxi, p.sigy, p.wyl, p.wyh, p.wyi,
+                             -1.0, -1, -1, -1, -1, err);
+      if (rc) return fail(GLSU_ERR_DEVICE, "spot-check plan: %s", err.c_str());
+      it = h->wins.emplace(key, wp).first;
+    }
+    WffPlan& wp = it->second;
+    if ((size_t)Ww * Hw > h->win_cap) {
+      hipFree(h->d_zwin);
+      hipFree(h->d_fwin);
+      h->d_zwin = h->d_fwin = nullptr;
+      h->win_cap = (size_t)Ww * Hw;
+      HIP_TRY(hipMalloc((void**)&h->d_zwin, h->win_cap * sizeof(float2)));
+      HIP_TRY(hipMalloc((void**)&h->d_fwin, h->win_cap * sizeof(float2)));
+    }
+    if (!h->d_res) {
+      HIP_TRY(hipMalloc((void**)&h->d_res, sizeof(float)));
+      HIP_TRY(hipHostMalloc((void**)&h->h_res, sizeof(float)));
+    }
+    // the window uses the full image's threshold
+    HIP_TRY(hipMemcpyAsync(wp.d_thr, p.d_thr, sizeof(float), hipMemcpyDeviceToDevice, h->s));
+    wp.thr_ready = true;
+    HIP_TRY(hipMemcpy2DAsync(h->d_zwin, Ww * sizeof(float2), h->z + (size_t)y0 * W + x0,
+                             W * sizeof(float2), Ww * sizeof(float2), Hw,
+                             hipMemcpyDeviceToDevice, h->s));
+    HIP_TRY(launch_rows_fwd(wp, h->d_zwin, h->s));
+    HIP_TRY(launch_cols(wp, h->s));
+    HIP_TRY(launch_rows_inv_filtered(wp, h->d_fwin, h->s));
+    HIP_TRY(launch_tile_residual(h->d_net, W, tx, ty, h->d_fwin, Ww, tx - x0, ty - y0, h->d_res,
+                                 h->s));
+    HIP_TRY(hipMemcpyAsync(h->h_res, h->d_res, sizeof(float), hipMemcpyDeviceToHost, h->s));
+    HIP_TRY(hipStreamSynchronize(h->s));
+    res = std::sqrt((double)*h->h_res / 4096.0);
+  }
+  const bool fb = tol >= 0 && res > tol;
+  UpdateArgs u{h->phiref, h->dref_hi, h->dref_lo, h->phicurr, h->dphi_hi, h->dphi_lo, d_dphi,
+               nullptr};
+  if (fb) {
+    HIP_TRY(launch_rows_fwd(p, h->z, h->s));
+    HIP_TRY(launch_cols(p, h->s));
+    HIP_TRY(launch_rows_inv_update(p, u, h->s));
+  } else {
+    HIP_TRY(launch_net_update(h->d_net, u, n, h->s));
+  }
+  h->fresh = true;
+  if (residual) *residual = res;
+  if (fallback) *fallback = fb ? 1 : 0;
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_copy_net(glsu_dpra* h, void* d_out) {
+  if (!h || !d_out) return fail(GLSU_ERR_ARG, "null argument");
+  if (!h->d_net) return fail(GLSU_ERR_ARG, "no learned frame has run");
+  HIP_TRY(hipMemcpyAsync(d_out, h->d_net, (size_t)h->W * h->H * sizeof(float2),
+                         hipMemcpyDeviceToDevice, h->s));
   return GLSU_OK;
 }

@@ -245,6 +245,54 @@ hipError_t launch_dpra_ls_batch(const BatchSeg* tab, int nseg, int ntiles, int g
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Learned path: phase update from the network output, and the tile residual.
+__global__ __launch_bounds__(256) void net_update_kernel(const float2* __restrict__ out,
+                                                         UpdateArgs u, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 o = out[i];
+  phase_update(u, i, atan2f(o.y, o.x));
+}
+
+hipError_t launch_net_update(const float2* out, const UpdateArgs& u, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(net_update_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out,
+                     u, n);
+  return hipGetLastError();
+}
+
+// sum over a 64 x 64 tile of wrap(angle(net) - angle(fwin))^2, added to *acc
+__global__ __launch_bounds__(256) void tile_residual_kernel(const float2* __restrict__ net, int W,
+                                                            int tx, int ty,
+                                                            const float2* __restrict__ fwin,
+                                                            int Ww, int ox, int oy,
+                                                            float* __restrict__ acc) {
+  __shared__ float red[256];
+  float s = 0.f;
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+    const int r = i / 64, c = i % 64;
+    const float2 a = net[(size_t)(ty + r) * W + tx + c];
+    const float2 b = fwin[(size_t)(oy + r) * Ww + ox + c];
+    // angle(a conj(b)) = wrapped phase difference
+    const float d = atan2f(a.y * b.x - a.x * b.y, a.x * b.x + a.y * b.y);
+    s += d * d;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *acc = red[0];
+}
+
+hipError_t launch_tile_residual(const float2* net, int W, int tx, int ty, const float2* fwin,
+                                int Ww, int ox, int oy, float* acc, hipStream_t s) {
+  hipLaunchKernelGGL(tile_residual_kernel, dim3(1), dim3(256), 0, s, net, W, tx, ty, fwin, Ww, ox,
+                     oy, acc);
+  return hipGetLastError();
+}
+
 int batch_grid_ls(int ntiles) {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess ||

@@ -79,6 +79,32 @@ struct UpdateArgs {
 };
 hipError_t launch_rows_inv_update(const WffPlan& p, const UpdateArgs& u, hipStream_t s);
 
+// DPRA phase update of pixel id from the filtered phase t = angle(F)
+// (DPRA/dpra_cpu.cpp:244-251): phicurr = wrap(phiref + t) into (-pi, pi]
+// with 2 pi split in two floats; dphi = dphiref + t carried as an unevaluated
+// float pair (TwoSum, then Fast2Sum renormalisation).
+__device__ __forceinline__ void phase_update(const UpdateArgs& u, size_t id, float t) {
+  float pc = u.phiref[id] + t;
+  if (pc > 3.14159265f) pc = (pc - 6.28318548f) + 1.74845553e-7f;
+  else if (pc <= -3.14159265f) pc = (pc + 6.28318548f) - 1.74845553e-7f;
+  u.phicurr[id] = pc;
+  const float hi = u.dref_hi[id], lo = u.dref_lo[id];
+  const float s = hi + t;
+  const float bp = s - hi;
+  const float err = (hi - (s - bp)) + (t - bp);
+  const float lo2 = lo + err;
+  const float nh = s + lo2;
+  const float nl = lo2 - (nh - s);
+  u.dphi_hi[id] = nh;
+  u.dphi_lo[id] = nl;
+  if (u.dphi_out) u.dphi_out[id] = nh;
+}
+// learned path: update from the network output (t = angle(out)); residual
+// of the network's phase against a WFF2 window on a 64 x 64 tile
+hipError_t launch_net_update(const float2* out, const UpdateArgs& u, size_t n, hipStream_t s);
+hipError_t launch_tile_residual(const float2* net, int W, int tx, int ty, const float2* fwin,
+                                int Ww, int ox, int oy, float* acc, hipStream_t s);
+
 // ---- batched launches over many images --------------------------------
 // One image of a batched launch (device-resident table entry).  row0 / col0 /
 // tile0 place the image in the item space of its plan class's row launches,

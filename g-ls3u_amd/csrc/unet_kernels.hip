@@ -1,0 +1,622 @@
+// unet_kernels.hip -- kernels of the learned WFF2 surrogate (a 4-level U-Net)
+// on gfx950: 3x3 convolutions as implicit GEMMs on BF16 MFMA
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation) for the forward pass, the
+// input gradient and the weight gradient, plus the small VALU kernels around
+// them (2-channel input layer, 1x1 output layer with the loss, max pooling,
+// upsampling adjoint, Adam).
+//
+// Layouts: activations NHWC bf16 (channels innermost, counts multiples of 32
+// except the 2-channel input/output); weights [Cout][9][Cin] bf16 for the
+// forward pass (K = tap*Cin + ci contiguous per output channel) and
+// [Cin][9][Cout] with flipped taps for the input gradient; FP32 master
+// weights, gradients and Adam moments in the [Cout][3][3][Cin] order.
+#include "unet_internal.h"
+
+#include <algorithm>
+
+namespace glsu {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+__device__ __forceinline__ bf16x8 zero8() {
+  bf16x8 z;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) z[i] = (__bf16)0.f;
+  return z;
+}
+
+// 8 channels [c, c+8) of pixel (n, y, x) of the (virtual) concatenation
+// cat(X1, up2(X2)); zero outside the image.  c is a multiple of 8 and a
+// 32-channel chunk never straddles the two sources (C1 % 32 == 0).
+__device__ __forceinline__ bf16x8 load8(const __bf16* __restrict__ X1, int C1,
+                                        const __bf16* __restrict__ X2, int C2, int n, int y,
+                                        int x, int H, int W, int c) {
+  if (y < 0 || y >= H || x < 0 || x >= W) return zero8();
+  if (c < C1) return *(const bf16x8*)(X1 + (((size_t)n * H + y) * W + x) * C1 + c);
+  const int H2 = H >> 1, W2 = W >> 1;
+  return *(const bf16x8*)(X2 + (((size_t)n * H2 + (y >> 1)) * W2 + (x >> 1)) * C2 + (c - C1));
+}
+
+// ------------------------------------------------------------------------
+// 3x3 convolution, implicit GEMM: M = pixels, N = output channels, K = 9*Cin.
+// Workgroup: 4 waves, output tile 4 rows x 64 pixels x NCO channels; wave w
+// owns row w.  Per 32-channel chunk the input tile (6 x 66 pixels) and the
+// chunk's weights (9 taps x NCO) are staged in LDS, then each wave issues
+// 4 x (NCO/16) MFMAs per tap.
+// MODE 0: y = relu(acc + bias).  MODE 1: y = acc * (mask > 0) (mask may be
+// null: y = acc).  Used for the forward pass (weights [Cout][9][Cin]) and
+// for the input gradient (dY as input, flipped transposed weights).
+constexpr int kPadC = 40;  // LDS pitch per pixel / output channel (bf16): 80 B
+constexpr int kTileW = 64, kTileH = 4;
+
+template <int NCO, int MODE>
+__global__ __launch_bounds__(256) void conv3_mfma_kernel(
+    const __bf16* __restrict__ X1, int C1, const __bf16* __restrict__ X2, int C2,
+    const __bf16* __restrict__ Wt, const float* __restrict__ bias,
+    const __bf16* __restrict__ mask, __bf16* __restrict__ Y, int n, int H, int W, int Cout) {
+  constexpr int NB = NCO / 16;
+  __shared__ __attribute__((aligned(16))) __bf16 sIn[(kTileH + 2) * (kTileW + 2) * kPadC];
+  __shared__ __attribute__((aligned(16))) __bf16 sW[9 * NCO * kPadC];
+  const int Cin = C1 + C2;
+  const int tilesx = (W + kTileW - 1) / kTileW;
+  const int tilesy = H / kTileH;
+  const int tile = blockIdx.x;
+  const int img = tile / (tilesx * tilesy);
+  const int ty = (tile / tilesx) % tilesy, tx = tile % tilesx;
+  const int y0 = ty * kTileH, x0 = tx * kTileW;
+  const int co0 = blockIdx.y * NCO;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int lr = lane % 16, lk = lane / 16;
+
+  floatx4 acc[4][NB];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
+    __syncthreads();  // previous chunk fully consumed
+    // input tile: (kTileH + 2) x (kTileW + 2) pixels x 4 groups of 8 channels
+    for (int i = threadIdx.x; i < (kTileH + 2) * (kTileW + 2) * 4; i += 256) {
+      const int g = i & 3, pix = i >> 2;
+      const int ry = pix / (kTileW + 2), rx = pix % (kTileW + 2);
+      *(bf16x8*)(sIn + pix * kPadC + g * 8) =
+          load8(X1, C1, X2, C2, img, y0 + ry - 1, x0 + rx - 1, H, W, ci0 + g * 8);
+    }
+    // weights of this chunk: 9 taps x NCO output channels x 32 channels
+    for (int i = threadIdx.x; i < 9 * NCO * 4; i += 256) {
+      const int g = i & 3, r = i >> 2;
+      const int t = r / NCO, co = r % NCO;
+      bf16x8 v = zero8();
+      if (co0 + co < Cout)
+        v = *(const bf16x8*)(Wt + ((size_t)(co0 + co) * 9 + t) * Cin + ci0 + g * 8);
+      *(bf16x8*)(sW + (t * NCO + co) * kPadC + g * 8) = v;
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int t = 0; t < 9; ++t) {
+      const int dy = t / 3, dx = t % 3;  // offsets + 1
+      bf16x8 a[4], b[NB];
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb)
+        a[mb] = *(const bf16x8*)(sIn + ((wave + dy) * (kTileW + 2) + mb * 16 + lr + dx) * kPadC +
+                                 lk * 8);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+        b[nb] = *(const bf16x8*)(sW + (t * NCO + nb * 16 + lr) * kPadC + lk * 8);
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+          acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb], b[nb], acc[mb][nb], 0, 0, 0);
+    }
+  }
+  // epilogue: lane holds rows m = 4*lk + j (pixels) of column lr (channel)
+  const int y = y0 + wave;
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const int co = co0 + nb * 16 + lr;
+    if (co >= Cout) continue;
+    const float bv = (MODE == 0) ? bias[co] : 0.f;
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int x = x0 + mb * 16 + 4 * lk + j;
+        if (x >= W) continue;
+        const size_t o = (((size_t)img * H + y) * W + x) * Cout + co;
+        float v = acc[mb][nb][j];
+        if (MODE == 0) {
+          v = fmaxf(v + bv, 0.f);
+        } else if (mask) {
+          v = ((float)mask[o] > 0.f) ? v : 0.f;
+        }
+        Y[o] = (__bf16)v;
+      }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Weight gradient of a 3x3 convolution: dW[co][t][ci] += sum_p dY[p][co] *
+// X[p + off_t][ci], one tap per workgroup (blockIdx.z), a 64 x 64 (co, ci)
+// tile (blockIdx.y) and a range of pixels (blockIdx.x, split K; partial sums
+// are added to the FP32 gradient atomically).  32-pixel chunks of dY and of
+// the shifted input are staged transposed in LDS ([channel][pixel]) so that
+// both MFMA operands are 16-byte LDS reads of 8 consecutive pixels.
+constexpr int kDwPix = 32, kDwPad = 40;
+
+__global__ __launch_bounds__(256) void conv3_dw_kernel(
+    const __bf16* __restrict__ dY, int Cout, const __bf16* __restrict__ X1, int C1,
+    const __bf16* __restrict__ X2, int C2, float* __restrict__ dW, int n, int H, int W,
+    int pix_per_block) {
+  __shared__ __attribute__((aligned(16))) __bf16 sA[64 * kDwPad];  // [co][pixel]
+  __shared__ __attribute__((aligned(16))) __bf16 sB[64 * kDwPad];  // [ci][pixel]
+  const int Cin = C1 + C2;
+  const int t = blockIdx.z, dy = t / 3 - 1, dx = t % 3 - 1;
+  const int nci = (Cin + 63) / 64;
+  const int co0 = (blockIdx.y / nci) * 64, ci0 = (blockIdx.y % nci) * 64;
+  const long P = (long)n * H * W;
+  const long p0 = (long)blockIdx.x * pix_per_block;
+  const long p1 = std::min(P, p0 + pix_per_block);
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int lr = lane % 16, lk = lane / 16;
+  const int wm = wave / 2, wn = wave % 2;  // wave tile: co [32 wm, +32), ci [32 wn, +32)
+  floatx4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  // staging role of this thread: pixel sp of the chunk, 8-channel group sg
+  const int sp = threadIdx.x / 8, sg = threadIdx.x % 8;
+  for (long pc = p0; pc < p1; pc += kDwPix) {
+    const long p = pc + sp;
+    bf16x8 va = zero8(), vb = zero8();
+    if (p < p1) {
+      const int x = (int)(p % W), y = (int)((p / W) % H), img = (int)(p / ((long)W * H));
+      const int co = co0 + sg * 8, ci = ci0 + sg * 8;
+      if (co < Cout) va = *(const bf16x8*)(dY + (size_t)p * Cout + co);
+      if (ci < Cin) vb = load8(X1, C1, X2, C2, img, y + dy, x + dx, H, W, ci);
+    }
+    __syncthreads();  // previous chunk consumed
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      sA[(sg * 8 + i) * kDwPad + sp] = va[i];
+      sB[(sg * 8 + i) * kDwPad + sp] = vb[i];
+    }
+    __syncthreads();
+    bf16x8 a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      a[i] = *(const bf16x8*)(sA + (wm * 32 + i * 16 + lr) * kDwPad + lk * 8);
+      b[i] = *(const bf16x8*)(sB + (wn * 32 + i * 16 + lr) * kDwPad + lk * 8);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+  // acc[i][j][r]: co = co0 + 32 wm + 16 i + 4 lk + r, ci = ci0 + 32 wn + 16 j + lr
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int co = co0 + 32 * wm + 16 * i + 4 * lk + r;
+        const int ci = ci0 + 32 * wn + 16 * j + lr;
+        if (co < Cout && ci < Cin) atomicAdd(dW + ((size_t)co * 9 + t) * Cin + ci, acc[i][j][r]);
+      }
+}
+
+// ------------------------------------------------------------------------
+// Input layer: 2 channels (x = (cos, sin), float2) -> C0 channels, 3x3, relu.
+// One thread per pixel and 8 output channels; weights [C0][9][2] in LDS.
+__global__ __launch_bounds__(256) void conv3_in_kernel(const float2* __restrict__ x,
+                                                       const float* __restrict__ w,
+                                                       const float* __restrict__ b,
+                                                       __bf16* __restrict__ Y, int n, int H,
+                                                       int W, int C0) {
+  extern __shared__ float sw[];  // C0 * 18 weights then C0 biases
+  for (int i = threadIdx.x; i < C0 * 19; i += blockDim.x) sw[i] = i < C0 * 18 ? w[i] : b[i - C0 * 18];
+  __syncthreads();
+  const int groups = C0 / 8;
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long P = (long)n * H * W;
+  if (gid >= P * groups) return;
+  const long p = gid / groups;
+  const int g = (int)(gid % groups);
+  const int xx = (int)(p % W), yy = (int)((p / W) % H), img = (int)(p / ((long)W * H));
+  float in[18];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    const int y2 = yy + t / 3 - 1, x2 = xx + t % 3 - 1;
+    float2 v = make_float2(0.f, 0.f);
+    if (y2 >= 0 && y2 < H && x2 >= 0 && x2 < W) v = x[((size_t)img * H + y2) * W + x2];
+    in[2 * t] = v.x;
+    in[2 * t + 1] = v.y;
+  }
+  bf16x8 o;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int co = g * 8 + k;
+    float s = sw[C0 * 18 + co];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) s = fmaf(sw[co * 18 + i], in[i], s);
+    o[k] = (__bf16)fmaxf(s, 0.f);
+  }
+  *(bf16x8*)(Y + (size_t)p * C0 + g * 8) = o;
+}
+
+// weight/bias gradient of the input layer: dW[co][t][c] = sum_p dY[p][co] x[p+off_t][c]
+__global__ __launch_bounds__(1024) void conv3_in_dw_kernel(const float2* __restrict__ x,
+                                                          const __bf16* __restrict__ dY,
+                                                          float* __restrict__ dW,
+                                                          float* __restrict__ db, int n, int H,
+                                                          int W, int C0, int pix_per_block) {
+  // thread k < C0*19 owns one weight (or bias) and sums it over the block's pixels
+  const int k = threadIdx.x;
+  const long P = (long)n * H * W;
+  const long p0 = (long)blockIdx.x * pix_per_block, p1 = std::min(P, p0 + pix_per_block);
+  if (k >= C0 * 19) return;
+  const bool isb = k >= C0 * 18;
+  const int co = isb ? k - C0 * 18 : k / 18;
+  const int t = (k % 18) / 2, c = k % 2;
+  float s = 0.f;
+  for (long p = p0; p < p1; ++p) {
+    const float g = (float)dY[(size_t)p * C0 + co];
+    if (isb) {
+      s += g;
+      continue;
+    }
+    const int xx = (int)(p % W), yy = (int)((p / W) % H), img = (int)(p / ((long)W * H));
+    const int y2 = yy + t / 3 - 1, x2 = xx + t % 3 - 1;
+    if (y2 >= 0 && y2 < H && x2 >= 0 && x2 < W) {
+      const float2 v = x[((size_t)img * H + y2) * W + x2];
+      s = fmaf(g, c ? v.y : v.x, s);
+    }
+  }
+  atomicAdd(isb ? db + co : dW + k, s);
+}
+
+// ------------------------------------------------------------------------
+// Output layer: out = x + W d + b (1x1, C0 -> 2), FP32.
+__global__ __launch_bounds__(256) void conv1_out_kernel(const __bf16* __restrict__ d,
+                                                        const float2* __restrict__ x,
+                                                        const float* __restrict__ w,
+                                                        const float* __restrict__ b,
+                                                        float2* __restrict__ out, long P, int C0) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  float s0 = b[0], s1 = b[1];
+  for (int c = 0; c < C0; c += 8) {
+    const bf16x8 v = *(const bf16x8*)(d + (size_t)p * C0 + c);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float f = (float)v[k];
+      s0 = fmaf(w[c + k], f, s0);
+      s1 = fmaf(w[C0 + c + k], f, s1);
+    }
+  }
+  const float2 xi = x[p];
+  out[p] = make_float2(xi.x + s0, xi.y + s1);
+}
+
+// Loss L = mean |out - u|^2 and its backward through the output layer:
+// g = 2 (out - u) / P; dd = (W^T g) * (d > 0); dW += g d^T; db += g.
+// Per-block partial sums of the loss and of dW/db go out through atomics.
+__global__ __launch_bounds__(256) void conv1_out_bwd_kernel(
+    const __bf16* __restrict__ d, const float2* __restrict__ out, const float2* __restrict__ u,
+    const float* __restrict__ w, __bf16* __restrict__ dd, float* __restrict__ dW,
+    float* __restrict__ db, float* __restrict__ loss, long P, int C0) {
+  extern __shared__ float red[];  // (2*C0 + 3) x 256 partials would be large: reduce per warp
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const float invP = 1.f / (float)P;
+  float g0 = 0.f, g1 = 0.f, l = 0.f;
+  if (p < P) {
+    const float2 o = out[p], t = u[p];
+    const float r0 = o.x - t.x, r1 = o.y - t.y;
+    l = (r0 * r0 + r1 * r1) * invP;
+    g0 = 2.f * r0 * invP;
+    g1 = 2.f * r1 * invP;
+  }
+  // loss and bias gradients: block reduction
+  float* s = red;
+  s[threadIdx.x] = l;
+  s[256 + threadIdx.x] = g0;
+  s[512 + threadIdx.x] = g1;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s[threadIdx.x] += s[threadIdx.x + o];
+      s[256 + threadIdx.x] += s[256 + threadIdx.x + o];
+      s[512 + threadIdx.x] += s[512 + threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(loss, s[0]);
+    atomicAdd(db, s[256]);
+    atomicAdd(db + 1, s[512]);
+  }
+  __syncthreads();
+  // per channel: dd and the dW partial (block-reduced through LDS)
+  for (int c = 0; c < C0; ++c) {
+    float f = 0.f;
+    if (p < P) {
+      f = (float)d[(size_t)p * C0 + c];
+      const float gd = w[c] * g0 + w[C0 + c] * g1;
+      dd[(size_t)p * C0 + c] = (__bf16)(f > 0.f ? gd : 0.f);
+    }
+    s[threadIdx.x] = g0 * f;
+    s[256 + threadIdx.x] = g1 * f;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) {
+        s[threadIdx.x] += s[threadIdx.x + o];
+        s[256 + threadIdx.x] += s[256 + threadIdx.x + o];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      atomicAdd(dW + c, s[0]);
+      atomicAdd(dW + C0 + c, s[256]);
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void maxpool2_kernel(const __bf16* __restrict__ x,
+                                                       __bf16* __restrict__ y, int n, int H,
+                                                       int W, int C) {
+  const int H2 = H / 2, W2 = W / 2, G = C / 8;
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)n * H2 * W2 * G) return;
+  const int g = (int)(gid % G);
+  const long q = gid / G;
+  const int x2 = (int)(q % W2), y2 = (int)((q / W2) % H2), img = (int)(q / ((long)W2 * H2));
+  const __bf16* b = x + (((size_t)img * H + 2 * y2) * W + 2 * x2) * C + g * 8;
+  const bf16x8 v0 = *(const bf16x8*)b, v1 = *(const bf16x8*)(b + C);
+  const bf16x8 v2 = *(const bf16x8*)(b + (size_t)W * C), v3 = *(const bf16x8*)(b + (size_t)W * C + C);
+  bf16x8 o;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    o[k] = (__bf16)fmaxf(fmaxf((float)v0[k], (float)v1[k]), fmaxf((float)v2[k], (float)v3[k]));
+  *(bf16x8*)(y + (size_t)q * C + g * 8) = o;
+}
+
+// Gradient into an encoder output e (C channels, H x W) from its two
+// consumers, then through its relu:  g_e = (pool adjoint of gp (H/2 x W/2)
+// routed to the first maximum of each window, row-major) + gskip (channels
+// [0, C) of the decoder's concatenated input gradient, pitch Cs; may be
+// null), times (e > 0).
+__global__ __launch_bounds__(256) void pool_bwd_kernel(const __bf16* __restrict__ e,
+                                                       const __bf16* __restrict__ gp,
+                                                       const __bf16* __restrict__ gskip, int Cs,
+                                                       __bf16* __restrict__ ge, int n, int H,
+                                                       int W, int C) {
+  const int H2 = H / 2, W2 = W / 2;
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)n * H2 * W2 * C) return;
+  const int c = (int)(gid % C);
+  const long q = gid / C;
+  const int x2 = (int)(q % W2), y2 = (int)((q / W2) % H2), img = (int)(q / ((long)W2 * H2));
+  const float g = gp ? (float)gp[q * C + c] : 0.f;
+  float best = -1.f;
+  int arg = 0;
+  size_t idx[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int yy = 2 * y2 + k / 2, xx = 2 * x2 + k % 2;
+    const size_t pix = ((size_t)img * H + yy) * W + xx;
+    idx[k] = pix;
+    const float v = (float)e[pix * C + c];
+    if (k == 0 || v > best) {
+      best = v;
+      arg = k;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const size_t pix = idx[k];
+    float v = (k == arg) ? g : 0.f;
+    if (gskip) v += (float)gskip[pix * Cs + c];
+    ge[pix * C + c] = (__bf16)(((float)e[pix * C + c] > 0.f) ? v : 0.f);
+  }
+}
+
+// Adjoint of the nearest x2 upsampling of a decoder input (channels [C1,
+// C1+C) of the concatenated gradient gcat, full resolution, pitch C1 + C),
+// times (a > 0) for the low-resolution activation a (C channels).
+__global__ __launch_bounds__(256) void up_bwd_kernel(const __bf16* __restrict__ gcat, int C1,
+                                                     const __bf16* __restrict__ a,
+                                                     __bf16* __restrict__ ga, int n, int H2,
+                                                     int W2, int C) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)n * H2 * W2 * C) return;
+  const int c = (int)(gid % C);
+  const long q = gid / C;
+  const int x2 = (int)(q % W2), y2 = (int)((q / W2) % H2), img = (int)(q / ((long)W2 * H2));
+  const int H = 2 * H2, W = 2 * W2, Cc = C1 + C;
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const size_t pix = ((size_t)img * H + 2 * y2 + k / 2) * W + 2 * x2 + k % 2;
+    s += (float)gcat[pix * Cc + C1 + c];
+  }
+  ga[q * C + c] = (__bf16)(((float)a[q * C + c] > 0.f) ? s : 0.f);
+}
+
+// db[c] += sum_p g[p][c]
+__global__ __launch_bounds__(256) void bias_grad_kernel(const __bf16* __restrict__ g,
+                                                        float* __restrict__ db, long P, int C,
+                                                        int pix_per_block) {
+  const int c = threadIdx.x;
+  if (c >= C) return;
+  const long p0 = (long)blockIdx.x * pix_per_block, p1 = std::min(P, p0 + pix_per_block);
+  float s = 0.f;
+  for (long p = p0; p < p1; ++p) s += (float)g[(size_t)p * C + c];
+  atomicAdd(db + c, s);
+}
+
+// Adam on FP32 master weights; refreshes the BF16 copies: Wf [Cout][9][Cin]
+// and, for 3x3 layers, the flipped transpose Wt [Cin][9][Cout].
+__global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ w,
+                                                   const float* __restrict__ g,
+                                                   float* __restrict__ m, float* __restrict__ v,
+                                                   long count, float lr, float b1, float b2,
+                                                   float eps, float c1, float c2) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+__global__ __launch_bounds__(256) void weights_to_bf16_kernel(const float* __restrict__ w,
+                                                              __bf16* __restrict__ wf,
+                                                              __bf16* __restrict__ wt, int Cout,
+                                                              int Cin) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)Cout * 9 * Cin) return;
+  const int ci = (int)(i % Cin), t = (int)((i / Cin) % 9), co = (int)(i / (9L * Cin));
+  const __bf16 h = (__bf16)w[i];
+  wf[i] = h;
+  if (wt) wt[((size_t)ci * 9 + (8 - t)) * Cout + co] = h;
+}
+
+__global__ void fill_kernel(float* __restrict__ p, long count, float v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+
+// ======================================================== host launchers
+static unsigned blocks(long work, int per = 256) { return (unsigned)((work + per - 1) / per); }
+
+hipError_t unet_conv3(const __bf16* X1, int C1, const __bf16* X2, int C2, const __bf16* Wt,
+                      const float* bias, const __bf16* mask, __bf16* Y, int n, int H, int W,
+                      int Cout, int mode, hipStream_t s) {
+  if ((C1 % 32) || (C2 % 32) || (H % kTileH) || (C2 && ((H | W) & 1))) return hipErrorInvalidValue;
+  const int tiles = n * (H / kTileH) * ((W + kTileW - 1) / kTileW);
+  const int nco = Cout >= 64 ? 64 : 32;
+  dim3 grid(tiles, (Cout + nco - 1) / nco);
+  if (nco == 64) {
+    if (mode == 0)
+      hipLaunchKernelGGL((conv3_mfma_kernel<64, 0>), grid, dim3(256), 0, s, X1, C1, X2, C2, Wt,
+                         bias, mask, Y, n, H, W, Cout);
+    else
+      hipLaunchKernelGGL((conv3_mfma_kernel<64, 1>), grid, dim3(256), 0, s, X1, C1, X2, C2, Wt,
+                         bias, mask, Y, n, H, W, Cout);
+  } else {
+    if (mode == 0)
+      hipLaunchKernelGGL((conv3_mfma_kernel<32, 0>), grid, dim3(256), 0, s, X1, C1, X2, C2, Wt,
+                         bias, mask, Y, n, H, W, Cout);
+    else
+      hipLaunchKernelGGL((conv3_mfma_kernel<32, 1>), grid, dim3(256), 0, s, X1, C1, X2, C2, Wt,
+                         bias, mask, Y, n, H, W, Cout);
+  }
+  return hipGetLastError();
+}
+
+hipError_t unet_conv3_dw(const __bf16* dY, int Cout, const __bf16* X1, int C1, const __bf16* X2,
+                         int C2, float* dW, int n, int H, int W, hipStream_t s) {
+  if ((C1 % 32) || (C2 % 32) || (Cout % 32)) return hipErrorInvalidValue;
+  const long P = (long)n * H * W;
+  const int ppb = 4096;
+  const int tilesc = ((Cout + 63) / 64) * ((C1 + C2 + 63) / 64);
+  dim3 grid((unsigned)((P + ppb - 1) / ppb), tilesc, 9);
+  hipLaunchKernelGGL(conv3_dw_kernel, grid, dim3(256), 0, s, dY, Cout, X1, C1, X2, C2, dW, n, H,
+                     W, ppb);
+  return hipGetLastError();
+}
+
+hipError_t unet_conv3_in(const float2* x, const float* w, const float* b, __bf16* Y, int n,
+                         int H, int W, int C0, hipStream_t s) {
+  const long work = (long)n * H * W * (C0 / 8);
+  hipLaunchKernelGGL(conv3_in_kernel, dim3(blocks(work)), dim3(256), C0 * 19 * sizeof(float), s,
+                     x, w, b, Y, n, H, W, C0);
+  return hipGetLastError();
+}
+
+hipError_t unet_conv3_in_dw(const float2* x, const __bf16* dY, float* dW, float* db, int n,
+                            int H, int W, int C0, hipStream_t s) {
+  if (C0 * 19 > 1024) return hipErrorInvalidValue;
+  const long P = (long)n * H * W;
+  const int ppb = 2048;
+  const int threads = ((C0 * 19 + 63) / 64) * 64;
+  hipLaunchKernelGGL(conv3_in_dw_kernel, dim3(blocks(P, ppb)), dim3(threads), 0, s, x, dY, dW,
+                     db, n, H, W, C0, ppb);
+  return hipGetLastError();
+}
+
+hipError_t unet_conv1_out(const __bf16* d, const float2* x, const float* w, const float* b,
+                          float2* out, long P, int C0, hipStream_t s) {
+  hipLaunchKernelGGL(conv1_out_kernel, dim3(blocks(P)), dim3(256), 0, s, d, x, w, b, out, P, C0);
+  return hipGetLastError();
+}
+
+hipError_t unet_conv1_out_bwd(const __bf16* d, const float2* out, const float2* u,
+                              const float* w, __bf16* dd, float* dW, float* db, float* loss,
+                              long P, int C0, hipStream_t s) {
+  hipLaunchKernelGGL(conv1_out_bwd_kernel, dim3(blocks(P)), dim3(256), 3 * 256 * sizeof(float),
+                     s, d, out, u, w, dd, dW, db, loss, P, C0);
+  return hipGetLastError();
+}
+
+hipError_t unet_maxpool2(const __bf16* x, __bf16* y, int n, int H, int W, int C, hipStream_t s) {
+  const long work = (long)n * (H / 2) * (W / 2) * (C / 8);
+  hipLaunchKernelGGL(maxpool2_kernel, dim3(blocks(work)), dim3(256), 0, s, x, y, n, H, W, C);
+  return hipGetLastError();
+}
+
+hipError_t unet_pool_bwd(const __bf16* e, const __bf16* gp, const __bf16* gskip, int Cs,
+                         __bf16* ge, int n, int H, int W, int C, hipStream_t s) {
+  const long work = (long)n * (H / 2) * (W / 2) * C;
+  hipLaunchKernelGGL(pool_bwd_kernel, dim3(blocks(work)), dim3(256), 0, s, e, gp, gskip, Cs, ge,
+                     n, H, W, C);
+  return hipGetLastError();
+}
+
+hipError_t unet_up_bwd(const __bf16* gcat, int C1, const __bf16* a, __bf16* ga, int n, int H2,
+                       int W2, int C, hipStream_t s) {
+  const long work = (long)n * H2 * W2 * C;
+  hipLaunchKernelGGL(up_bwd_kernel, dim3(blocks(work)), dim3(256), 0, s, gcat, C1, a, ga, n, H2,
+                     W2, C);
+  return hipGetLastError();
+}
+
+hipError_t unet_bias_grad(const __bf16* g, float* db, long P, int C, hipStream_t s) {
+  const int ppb = 1024;
+  hipLaunchKernelGGL(bias_grad_kernel, dim3(blocks(P, ppb)), dim3(((C + 63) / 64) * 64), 0, s, g,
+                     db, P, C, ppb);
+  return hipGetLastError();
+}
+
+hipError_t unet_adam(float* w, const float* g, float* m, float* v, long count, float lr,
+                     float b1, float b2, float eps, int step, hipStream_t s) {
+  const float c1 = 1.f / (1.f - powf(b1, (float)step));
+  const float c2 = 1.f / (1.f - powf(b2, (float)step));
+  hipLaunchKernelGGL(adam_kernel, dim3(blocks(count)), dim3(256), 0, s, w, g, m, v, count, lr,
+                     b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t unet_weights_bf16(const float* w, __bf16* wf, __bf16* wt, int Cout, int Cin,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(weights_to_bf16_kernel, dim3(blocks((long)Cout * 9 * Cin)), dim3(256), 0, s,
+                     w, wf, wt, Cout, Cin);
+  return hipGetLastError();
+}
+
+hipError_t unet_fill(float* p, long count, float v, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_kernel, dim3(blocks(count)), dim3(256), 0, s, p, count, v);
+  return hipGetLastError();
+}
+
+}  // namespace glsu

@@ -258,23 +258,7 @@ __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ D,
       filtered[id] = F;
     } else {
       if (u.filtered) u.filtered[id] = F;
-      const float t = atan2f(F.y, F.x);
-      // wrap(phiref + t) into (-pi, pi]; 2*pi split in two floats
-      float pc = u.phiref[id] + t;
-      if (pc > 3.14159265f) pc = (pc - 6.28318548f) + 1.74845553e-7f;
-      else if (pc <= -3.14159265f) pc = (pc + 6.28318548f) - 1.74845553e-7f;
-      u.phicurr[id] = pc;
-      // dphi = dphiref + t, carried as an unevaluated float pair (TwoSum)
-      const float hi = u.dref_hi[id], lo = u.dref_lo[id];
-      const float s = hi + t;
-      const float bp = s - hi;
-      const float err = (hi - (s - bp)) + (t - bp);
-      const float lo2 = lo + err;
-      const float nh = s + lo2;                // renormalise (Fast2Sum)
-      const float nl = lo2 - (nh - s);
-      u.dphi_hi[id] = nh;
-      u.dphi_lo[id] = nl;
-      if (u.dphi_out) u.dphi_out[id] = nh;
+      phase_update(u, id, atan2f(F.y, F.x));
     }
   }
 }

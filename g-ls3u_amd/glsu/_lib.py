@@ -135,6 +135,18 @@ def lib():
         "glsu_dpra_batch_get_tuning": (I, [V, I, PT]),
         "glsu_dpra_batch_get_state": (I, [V, I, V, V]),
         "glsu_dpra_batch_destroy": (I, [V]),
+        "glsu_unet_create": (I, [PV, I, I, ctypes.c_ulonglong, V]),
+        "glsu_unet_param_count": (I, [V, ctypes.POINTER(ctypes.c_long)]),
+        "glsu_unet_get_params": (I, [V, V]),
+        "glsu_unet_set_params": (I, [V, V]),
+        "glsu_unet_get_grads": (I, [V, V]),
+        "glsu_unet_forward": (I, [V, V, V, I, I, I]),
+        "glsu_unet_loss_grad": (I, [V, V, V, I, I, I, PD]),
+        "glsu_unet_adam_step": (I, [V, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_float]),
+        "glsu_unet_destroy": (I, [V]),
+        "glsu_dpra_frame_learned": (I, [V, V, V, I, V, D, I, I, PD, PI]),
+        "glsu_dpra_copy_net": (I, [V, V]),
         "glsu_fft_plans": (I, [PI, PI, PI, PI, I, PI]),
     }
     for name, (res, args) in sig.items():
@@ -167,7 +179,10 @@ def exported_symbols():
         "glsu_dpra_profile", "glsu_dpra_time_kernel", "glsu_dpra_get_state", "glsu_dpra_set_state", "glsu_dpra_copy_z", "glsu_dpra_thr",
         "glsu_dpra_destroy", "glsu_gen_frames", "glsu_dpra_batch_create", "glsu_dpra_batch_run",
         "glsu_dpra_batch_info", "glsu_dpra_batch_get_tuning", "glsu_dpra_batch_get_state",
-        "glsu_dpra_batch_destroy", "glsu_fft_plans",
+        "glsu_dpra_batch_destroy", "glsu_unet_create", "glsu_unet_param_count",
+        "glsu_unet_get_params", "glsu_unet_set_params", "glsu_unet_get_grads", "glsu_unet_forward",
+        "glsu_unet_loss_grad", "glsu_unet_adam_step", "glsu_unet_destroy",
+        "glsu_dpra_frame_learned", "glsu_dpra_copy_net", "glsu_fft_plans",
     ]
 
 

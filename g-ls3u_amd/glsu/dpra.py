@@ -114,6 +114,27 @@ class DPRA:
         check(lib().glsu_dpra_time_kernel(self._h, int(stage), int(reps), ctypes.byref(ms)))
         return ms.value
 
+    def frame_learned(self, frame, net, dphi=None, tol=0.03, tile=(0, 0)):
+        """dpra_per_frame with the U-Net `net` (glsu.learned.UNet) in place of
+        WFF2 and a WFF2 spot check on the 64x64 tile at `tile` (x, y): if the
+        network's phase there differs from WFF2's by more than `tol` rad RMS,
+        the frame is filtered by WFF2 instead.  Returns (residual, fallback)."""
+        dt = self._check_frame(frame)
+        if dphi is None:
+            dphi = torch.empty((self.height, self.width), dtype=torch.float32, device=frame.device)
+        res = ctypes.c_double()
+        fb = ctypes.c_int()
+        check(lib().glsu_dpra_frame_learned(
+            self._h, net._h, ctypes.c_void_p(frame.data_ptr()), dt, ctypes.c_void_p(dphi.data_ptr()),
+            float(tol), int(tile[0]), int(tile[1]), ctypes.byref(res), ctypes.byref(fb)))
+        self.last_dphi = dphi
+        return res.value, bool(fb.value)
+
+    def last_net(self):
+        z = torch.empty((self.height, self.width), dtype=torch.complex64, device="cuda")
+        check(lib().glsu_dpra_copy_net(self._h, ctypes.c_void_p(z.data_ptr())))
+        return z
+
     def get_state(self):
         n = self.height * self.width
         phr = np.empty(n, np.float32)

@@ -1,0 +1,201 @@
+"""Learned WFF2 surrogate: a U-Net on BF16 MFMA kernels (libglsu), trained on
+(raw DPRA field, WFF2-filtered field) pairs that the plain method generates.
+
+The network maps z = exp(j Delta) (the raw DPRA phase difference, as
+cos/sin channels) to the unit vector of the WFF2-filtered field
+F = WFF2(z) (WFT/WFT2_CPU.cpp:190-390, called from DPRA/dpra_cpu.cpp:238).
+Loss: mean |out - F/|F||^2.  At run time DPRA.frame_learned replaces the
+WFF2 call by the network and keeps WFF2 as a per-frame spot check and
+fallback.  Forward, backward and Adam all run in libglsu's HIP kernels; torch
+only holds device memory.
+"""
+import ctypes
+import os
+
+import numpy as np
+import torch
+
+from ._lib import check, lib
+from .params import wff_params
+from . import synth
+
+
+def layer_specs(levels=4, base=32, cin=2):
+    """[(name, kind, Cin, Cout)] in parameter order (kind 'c3' or 'c1')."""
+    ch = [base << l for l in range(levels)]
+    specs, prev = [], cin
+    for l in range(levels):
+        specs.append((f"e{l}a", "c3", prev, ch[l]))
+        specs.append((f"e{l}b", "c3", ch[l], ch[l]))
+        prev = ch[l]
+    for l in range(levels - 2, -1, -1):
+        specs.append((f"d{l}a", "c3", ch[l] + ch[l + 1], ch[l]))
+        specs.append((f"d{l}b", "c3", ch[l], ch[l]))
+    specs.append(("out", "c1", ch[0], cin))
+    return specs
+
+
+class UNet:
+    def __init__(self, levels=4, base=32, seed=0, stream=None):
+        self.levels, self.base = int(levels), int(base)
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        h = ctypes.c_void_p()
+        check(lib().glsu_unet_create(ctypes.byref(h), self.levels, self.base, int(seed),
+                                     ctypes.c_void_p(self.stream.cuda_stream)))
+        self._h = h
+        c = ctypes.c_long()
+        check(lib().glsu_unet_param_count(h, ctypes.byref(c)))
+        self.count = c.value
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._h = None
+            try:
+                lib().glsu_unet_destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
+
+    __del__ = close
+
+    # ---- parameters -----------------------------------------------------
+    def get_flat(self, grads=False):
+        a = np.empty(self.count, np.float32)
+        fn = lib().glsu_unet_get_grads if grads else lib().glsu_unet_get_params
+        check(fn(self._h, a.ctypes.data))
+        return a
+
+    def set_flat(self, a):
+        a = np.ascontiguousarray(a, np.float32)
+        if a.size != self.count:
+            raise ValueError(f"expected {self.count} parameters, got {a.size}")
+        check(lib().glsu_unet_set_params(self._h, a.ctypes.data))
+
+    def _to_dict(self, flat):
+        out, i = {}, 0
+        for name, kind, ci, co in layer_specs(self.levels, self.base):
+            k = 3 if kind == "c3" else 1
+            nw = co * k * k * ci
+            out[name + ".w"] = flat[i:i + nw].reshape(co, k, k, ci)
+            i += nw
+            out[name + ".b"] = flat[i:i + co]
+            i += co
+        return out
+
+    def params(self):
+        return self._to_dict(self.get_flat())
+
+    def grads(self):
+        return self._to_dict(self.get_flat(grads=True))
+
+    def set_params(self, d):
+        flat = []
+        for name, kind, ci, co in layer_specs(self.levels, self.base):
+            flat.append(np.asarray(d[name + ".w"], np.float32).reshape(-1))
+            flat.append(np.asarray(d[name + ".b"], np.float32).reshape(-1))
+        self.set_flat(np.concatenate(flat))
+
+    def save(self, path):
+        np.savez(path, levels=self.levels, base=self.base, params=self.get_flat())
+
+    @staticmethod
+    def load(path, stream=None):
+        z = np.load(path)
+        net = UNet(int(z["levels"]), int(z["base"]), stream=stream)
+        net.set_flat(z["params"])
+        return net
+
+    # ---- compute --------------------------------------------------------
+    @staticmethod
+    def _c64(t, name):
+        if t.dtype != torch.complex64 or not t.is_cuda or not t.is_contiguous() or t.dim() != 3:
+            raise ValueError(f"{name} must be a contiguous complex64 CUDA tensor (n, H, W)")
+        return t
+
+    def forward(self, x, out=None):
+        self._c64(x, "x")
+        if out is None:
+            out = torch.empty_like(x)
+        n, H, W = x.shape
+        check(lib().glsu_unet_forward(self._h, ctypes.c_void_p(x.data_ptr()),
+                                      ctypes.c_void_p(out.data_ptr()), n, H, W))
+        return out
+
+    def loss_grad(self, x, u):
+        self._c64(x, "x")
+        self._c64(u, "u")
+        n, H, W = x.shape
+        loss = ctypes.c_double()
+        check(lib().glsu_unet_loss_grad(self._h, ctypes.c_void_p(x.data_ptr()),
+                                        ctypes.c_void_p(u.data_ptr()), n, H, W,
+                                        ctypes.byref(loss)))
+        return loss.value
+
+    def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        check(lib().glsu_unet_adam_step(self._h, float(lr), float(beta1), float(beta2),
+                                        float(eps)))
+
+
+# ------------------------------------------------------------------ data
+def make_pairs(nseq=8, nframes=25, size=1024, seed0=900000, params=None, log=None):
+    """(x, u) pairs from the plain method on synthetic sequences with held-out
+    seeds: x = raw DPRA field z of each frame, u = unit vector of WFF2(z).
+    Per sequence: peaks amplitude in U[2, 5] (+0.1 t), noise std in U[5, 25].
+    Returns two complex64 CUDA tensors (nseq*nframes, size, size)."""
+    from .dpra import DPRA, gen_frames
+    from .wft2 import WFT2
+    params = params if params is not None else wff_params()
+    rng = np.random.default_rng(seed0)
+    xs, us = [], []
+    w = WFT2(size, size, params=params)
+    for s in range(nseq):
+        amp0, noise = float(rng.uniform(2, 5)), float(rng.uniform(5, 25))
+        fr = gen_frames(size, size, nframes, seed0 + s, amp0=amp0, noise=noise)
+        d = DPRA(synth.phase(size, size, amp0, 0.05), params=params)
+        for t in range(nframes):
+            d.dpra_per_frame(fr[t])
+            d.update_ref_phi()
+            z = d.last_z()
+            F = w(z)
+            xs.append(z)
+            us.append(F / F.abs().clamp_min(1e-30))
+        if log:
+            log(f"pairs: sequence {s + 1}/{nseq} (amp0 {amp0:.2f}, noise {noise:.1f})")
+    return torch.stack(xs), torch.stack(us)
+
+
+def random_crops(x, u, batch, crop, gen):
+    """A batch of random crop x crop windows from the pair tensors."""
+    N, H, W = x.shape
+    idx = torch.randint(0, N, (batch,), generator=gen)
+    ys = torch.randint(0, H - crop + 1, (batch,), generator=gen)
+    xs = torch.randint(0, W - crop + 1, (batch,), generator=gen)
+    bx = torch.stack([x[i, a:a + crop, b:b + crop] for i, a, b in zip(idx.tolist(), ys.tolist(), xs.tolist())])
+    bu = torch.stack([u[i, a:a + crop, b:b + crop] for i, a, b in zip(idx.tolist(), ys.tolist(), xs.tolist())])
+    return bx.contiguous(), bu.contiguous()
+
+
+def train(net, x, u, crops=20000, batch=16, crop=256, lr=1e-3, seed=0, log=None, log_every=100):
+    """Adam on random crops of the pairs until `crops` crops have been seen.
+    Returns the list of per-step losses."""
+    gen = torch.Generator().manual_seed(seed)
+    steps = (crops + batch - 1) // batch
+    losses = []
+    for k in range(steps):
+        bx, bu = random_crops(x, u, batch, crop, gen)
+        # cosine learning-rate decay to 5 % of lr
+        lr_k = lr * (0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * k / max(steps - 1, 1))))
+        losses.append(net.loss_grad(bx, bu))
+        net.adam_step(lr_k)
+        if log and (k % log_every == 0 or k == steps - 1):
+            log(f"step {k + 1}/{steps}: loss {losses[-1]:.5f} lr {lr_k:.2e}")
+    return losses
+
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_l4_b32.npz")
+
+
+def phase_rms(a, b):
+    """RMS of the wrapped phase difference of two complex tensors (radians)."""
+    d = torch.angle(a * b.conj())
+    return float(torch.sqrt(torch.mean(d * d)))

@@ -203,6 +203,43 @@ int glsu_dpra_batch_get_tuning(glsu_dpra_batch *h, int seq, glsu_tuning *t);
 int glsu_dpra_batch_get_state(glsu_dpra_batch *h, int seq, float *phiref, float *dphiref);
 int glsu_dpra_batch_destroy(glsu_dpra_batch *h);
 
+/* ---- learned WFF2 surrogate ---------------------------------------------
+ * A U-Net (levels x base channels; 3x3 convs, relu, max pooling, nearest x2
+ * upsampling with skip concatenation, 1x1 output with a residual on the
+ * input) that maps the raw DPRA field z = (cos, sin) of the phase difference
+ * to the WFF2-filtered unit vector.  BF16 weights and activations on MFMA
+ * with FP32 accumulation; FP32 master weights and Adam.  No counterpart in
+ * the reference: it stands in for WFT2_cpu::operator() (WFT/WFT2_CPU.cpp:
+ * 178-186) inside dpra_per_frame, and WFF2 stays its fallback and judge.
+ * Parameters are flat FP32 in layer order (e0a e0b ... e{L-1}b d{L-2}a d{L-2}b
+ * ... d0a d0b out), each layer's weights [Cout][k][k][Cin] then biases.
+ * Images: n x height x width complex64, height a multiple of 4*2^(levels-1),
+ * width of 2^(levels-1). */
+typedef struct glsu_unet glsu_unet;
+int glsu_unet_create(glsu_unet **out, int levels, int base, unsigned long long seed,
+                     void *hip_stream);
+int glsu_unet_param_count(glsu_unet *h, long *count);
+int glsu_unet_get_params(glsu_unet *h, float *host);
+int glsu_unet_set_params(glsu_unet *h, const float *host);
+int glsu_unet_get_grads(glsu_unet *h, float *host);
+int glsu_unet_forward(glsu_unet *h, const void *d_x, void *d_out, int n, int height, int width);
+/* Loss L = mean |out - u|^2 (u: target unit vectors, complex64) and its
+ * gradients into the handle (read them with glsu_unet_get_grads). */
+int glsu_unet_loss_grad(glsu_unet *h, const void *d_x, const void *d_u, int n, int height,
+                        int width, double *loss);
+int glsu_unet_adam_step(glsu_unet *h, float lr, float beta1, float beta2, float eps);
+int glsu_unet_destroy(glsu_unet *h);
+/* dpra_per_frame with the network in place of WFF2 (DPRA/dpra_cpu.cpp:238),
+ * and a WFF2 spot check on the 64x64 tile at (tile_x, tile_y): when the RMS
+ * wrapped difference between the network's phase and WFF2's exceeds tol
+ * (radians), the frame is filtered by WFF2 instead (*fallback = 1).  tol < 0
+ * disables the check.  *residual receives the RMS (or -1). */
+int glsu_dpra_frame_learned(glsu_dpra *h, glsu_unet *net, const void *d_frame, int frame_dtype,
+                            float *d_dphi, double tol, int tile_x, int tile_y, double *residual,
+                            int *fallback);
+/* The network output (complex64, height*width) of the last learned frame. */
+int glsu_dpra_copy_net(glsu_dpra *h, void *d_out);
+
 /* ---- utilities ---------------------------------------------------------- */
 /* Seeded synthetic fringe frames, written to device memory (float32,
  * nframes*height*width): I = 128 + 100 cos(phi) + noise*N(0,1),

@@ -1,0 +1,137 @@
+"""Learned WFF2 surrogate on the GPU: forward pass and every parameter
+gradient of the BF16-MFMA kernels against the FP64 CPU reference
+(oracle/unet_ref.py, itself pinned by finite differences), training on GPU
+pairs, and the learned DPRA frame with its WFF2 spot check and fallback.
+
+Tolerances: BF16 weights and activations (8-bit mantissa, relative rounding
+3.9e-3 per value) through ~15 layers give a few 1e-3 to 1e-2 relative error;
+the bars are 3e-2 (forward, relative to the network's correction x - out) and
+6e-2 (per-layer relative gradient norm).  The 2-channel input layer's weight
+gradient is a sum over pixels of random-phase inputs that cancels to ~1/10 of
+its terms, which amplifies the BF16 error it inherits from the whole backward
+pass; it is held to direction (cosine > 0.98) and a 0.2 relative norm, which
+a wrong tap, channel or sign would fail by far."""
+import numpy as np
+import pytest
+
+from _util import say
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+import glsu  # noqa: E402
+from oracle import unet_ref as U  # noqa: E402
+
+
+def _bf16(a):
+    """Round to bfloat16 (nearest even), returned as float64."""
+    f = np.ascontiguousarray(a, np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return u.view(np.float32).astype(np.float64)
+
+
+def _net_and_ref(seed=3, levels=4, base=32):
+    net = glsu.UNet(levels, base, seed=seed)
+    p = net.params()
+    rng = np.random.default_rng(seed)
+    p["out.w"] = rng.normal(0, 0.2, p["out.w"].shape).astype(np.float32)
+    for k in p:
+        if k.endswith(".b"):
+            p[k] = rng.normal(0, 0.05, p[k].shape).astype(np.float32)
+    net.set_params(p)
+    # the CPU reference sees what the kernels use: BF16 weights of the MFMA
+    # layers, FP32 weights of the 2-channel input layer and the 1x1 output
+    ref = {}
+    for k, v in p.items():
+        mfma = k.endswith(".w") and not k.startswith("e0a") and not k.startswith("out")
+        ref[k] = _bf16(v) if mfma else v.astype(np.float64)
+    return net, ref
+
+
+def _inputs(n, H, W, seed=5):
+    rng = np.random.default_rng(seed)
+    t = rng.uniform(-np.pi, np.pi, (n, H, W))
+    tf = t + rng.normal(0, 0.3, t.shape)
+    x = np.exp(1j * t).astype(np.complex64)
+    u = np.exp(1j * tf).astype(np.complex64)
+    return x, u
+
+
+def _c2(a):
+    return np.stack([a.real, a.imag], -1).astype(np.float64)
+
+
+def test_forward_matches_cpu_reference():
+    net, ref = _net_and_ref()
+    x, _ = _inputs(2, 64, 96)
+    out = net.forward(torch.from_numpy(x).cuda()).cpu().numpy()
+    r = U.forward(ref, _c2(x))
+    corr_ref = r - _c2(x)
+    err = np.abs(_c2(out) - r).max() / np.abs(corr_ref).max()
+    rel = np.linalg.norm(_c2(out) - r) / np.linalg.norm(corr_ref)
+    say(f"[unet fwd] max err / max correction = {err:.2e}, relative norm {rel:.2e}")
+    assert rel < 3e-2
+
+
+def test_gradients_match_cpu_reference():
+    net, ref = _net_and_ref(seed=4)
+    x, u = _inputs(2, 64, 64, seed=6)
+    loss = net.loss_grad(torch.from_numpy(x).cuda(), torch.from_numpy(u).cuda())
+    g = net.grads()
+    Lr, gr = U.loss_and_grads(ref, _c2(x), _c2(u))
+    say(f"[unet grad] loss gpu {loss:.6f} cpu {Lr:.6f}")
+    assert abs(loss - Lr) / Lr < 1e-2
+    for k in gr:
+        a, b = g[k].astype(np.float64).ravel(), gr[k].ravel()
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        cos = float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30))
+        say(f"   {k:8s} |g|={np.linalg.norm(b):.3e} rel err {rel:.2e} cos {cos:.5f}")
+        if k.startswith("e0a"):
+            assert rel < 0.2 and cos > 0.98, k
+        else:
+            assert rel < 6e-2, k
+
+
+def test_training_reduces_loss():
+    x, u = glsu.learned.make_pairs(nseq=1, nframes=3, size=256, seed0=777)
+    net = glsu.UNet(4, 32, seed=1)
+    losses = glsu.learned.train(net, x, u, crops=4 * 80, batch=4, crop=128, lr=2e-3, seed=0,
+                                log=say, log_every=20)
+    first, last = np.mean(losses[:5]), np.mean(losses[-5:])
+    say(f"[unet train] loss {first:.4f} -> {last:.4f}")
+    assert np.isfinite(losses).all()
+    assert last < 0.7 * first
+
+
+def test_learned_frame_fallback_and_net_paths():
+    W = H = 256
+    frames = glsu.gen_frames(W, H, 2, 42)
+    phi0 = glsu.synth.phase(W, H, 3.0, 0.05)
+    net = glsu.UNet(4, 32, seed=2)
+    # plain reference
+    d0 = glsu.DPRA(phi0)
+    ref = d0(frames).cpu().numpy()
+    # tol = 0: every frame falls back to WFF2 -> identical to the plain chain
+    d1 = glsu.DPRA(phi0)
+    outs = []
+    for t in range(2):
+        res, fb = d1.frame_learned(frames[t], net, tol=0.0, tile=(96, 96))
+        d1.update_ref_phi()
+        outs.append(d1.last_dphi.cpu().numpy())
+        say(f"[learned] tol 0 frame {t}: residual {res:.3e} fallback {fb}")
+        assert fb and res > 0
+    assert np.array_equal(np.stack(outs), ref)
+    # untrained net = identity (out = z): no check -> the update uses angle(z)
+    d2 = glsu.DPRA(phi0)
+    res, fb = d2.frame_learned(frames[0], net, tol=-1.0)
+    assert res == -1.0 and not fb
+    z = d2.last_z()
+    zn = d2.last_net()
+    assert torch.equal(z, zn)
+    dphi = d2.last_dphi.cpu().numpy()
+    expect = np.angle(z.cpu().numpy())
+    assert np.abs(np.angle(np.exp(1j * (dphi - expect)))).max() < 1e-6

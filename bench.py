@@ -38,8 +38,107 @@ WORKLOADS = {
     "heavy2048": (2048, 50, (12.0, -0.5, 0.5, 0.05), (3.0, 0.1, 0.08, 20.0)),
     # 64 sequences, sizes uniform over {256..2048}^2, 30 frames each, batched
     "batch64": (None, 30, (10.0, -0.2, 0.2, 0.1), None),
+    # learned WFF2 surrogate: 1024^2, 500 frames, noise std drawn from U[5, 25]
+    "learned1024": (1024, 500, (10.0, -0.2, 0.2, 0.1), None),
 }
 BATCH_SIZES = [256, 512, 768, 1024, 1536, 2048]
+
+
+def bench_learned(args, glsu, torch, np):
+    """Learned filter vs plain WFF2 on one 1024^2 sequence (500 frames, noise
+    std drawn from U[5, 25], seeds disjoint from the training pairs').  value =
+    frames/s of the learned chain (U-Net in place of WFF2, with the 64x64 WFF2
+    spot check and fallback every frame); also the plain chain's frames/s on
+    the same frames, and on sampled frames the wrapped-phase RMS of the
+    network's filtered phase, and of the unfiltered DPRA phase, against the
+    FP64 WFF2 of the same raw field (the CPU oracle)."""
+    size, T, wff, _ = WORKLOADS["learned1024"]
+    W = H = args.size or size
+    T = args.frames or T
+    rng = np.random.default_rng(31337)
+    amp0, noise = float(rng.uniform(2, 5)), float(rng.uniform(5, 25))
+    params = glsu.wff_params(*wff)
+    frames = glsu.gen_frames(W, H, T, 424242, amp0=amp0, noise=noise)
+    phi0 = glsu.synth.phase(W, H, amp0, 0.05)
+    path = glsu.learned.WEIGHTS
+    trained_here = not os.path.exists(path)
+    if trained_here:  # no committed weights: a short training run (untimed)
+        x, u = glsu.learned.make_pairs(8, 25, 1024)
+        net = glsu.UNet(4, 32, seed=0)
+        glsu.learned.train(net, x, u, crops=8000)
+        del x, u
+    else:
+        net = glsu.UNet.load(path)
+    tol = 0.03
+    # plain chain, graph mode
+    d = glsu.DPRA(phi0, params=params)
+    out = torch.empty((T, H, W), dtype=torch.float32, device="cuda")
+    d(frames[:10].contiguous(), out=out[:10], graph=True)
+    d = glsu.DPRA(phi0, params=params)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d(frames, out=out, graph=True)
+    torch.cuda.synchronize()
+    plain_fps = T / (time.perf_counter() - t0)
+    # learned chain: per frame LS -> U-Net -> spot check (tile moves every frame)
+    def tile(t):
+        return ((t * 389) % (W - 64), (t * 613) % (H - 64))
+    dl = glsu.DPRA(phi0, params=params)
+    for t in range(3):
+        dl.frame_learned(frames[t], net, out[t], tol=tol, tile=tile(t))
+        dl.update_ref_phi()
+    dl = glsu.DPRA(phi0, params=params)
+    nfb, res = 0, []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(T):
+        r, fb = dl.frame_learned(frames[t], net, out[t], tol=tol, tile=tile(t))
+        dl.update_ref_phi()
+        nfb += int(fb)
+        res.append(r)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    learned_fps = T / dt
+    # accuracy on sampled frames against the FP64 WFF2 of the same raw field
+    acc = {}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        wf = O.WFF2(W, H, *wff, nthreads=min(os.cpu_count() or 1, 16))
+        dc = glsu.DPRA(phi0, params=params)
+        errs_net, errs_raw = [], []
+        sample = [0, T // 4, T // 2, T - 1]
+        for t in range(T):
+            dc.frame_learned(frames[t], net, tol=-1.0)
+            dc.update_ref_phi()
+            if t in sample:
+                z = dc.last_z().cpu().numpy().astype(np.complex128)
+                F = wf(z)
+                y = dc.last_net().cpu().numpy().astype(np.complex128)
+                en = np.angle(y * np.conj(F))
+                er = np.angle(z * np.conj(F))
+                errs_net.append(float(np.sqrt(np.mean(en * en))))
+                errs_raw.append(float(np.sqrt(np.mean(er * er))))
+        acc = {"frames": sample, "net_rms_rad": errs_net, "unfiltered_rms_rad": errs_raw,
+               "gain_over_unfiltered": round(float(np.mean(errs_raw) / np.mean(errs_net)), 2)}
+    line = {
+        "metric": METRIC, "value": round(learned_fps, 2), "unit": "frames/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (net, FP32 accumulate) + f32",
+        "data": "synthetic",
+        "config": {"workload": f"learned WFF2 surrogate, {W}x{H}, {T} frames: DPRA LS -> 4-level "
+                               "U-Net (32 base ch, BF16 MFMA) -> phase update, WFF2 on a "
+                               f"64x64 tile every frame, fallback above {tol} rad RMS; "
+                               f"noise std {noise:.2f}, amp0 {amp0:.2f}",
+                   "name": "learned1024", "weights": "trained in this run" if trained_here
+                   else os.path.relpath(path, ROOT)},
+        "plain_frames_per_s": round(plain_fps, 2),
+        "learned_over_plain": round(learned_fps / plain_fps, 3),
+        "fallback_frames": nfb,
+        "spot_check_rms_rad": {"mean": round(float(np.mean(res)), 5),
+                               "max": round(float(np.max(res)), 5)},
+        "accuracy_vs_fp64_wff2": acc,
+    }
+    print(json.dumps(line), flush=True)
 
 
 def bench_batch(args, glsu, torch, np):
@@ -195,6 +294,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.workload == "batch64":
         return bench_batch(args, glsu, torch, np)
+    if args.workload == "learned1024":
+        return bench_learned(args, glsu, torch, np)
     size, T, wff, gen = WORKLOADS[args.workload]
     W = H = args.size or size
     T = args.frames or T

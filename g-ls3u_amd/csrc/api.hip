@@ -1124,8 +1124,11 @@ extern "C" int glsu_dpra_frame_learned(glsu_dpra* h, glsu_unet* net, const void*
     if (it == h->wins.end()) {
       WffPlan wp;
       std::string err;
+      // same plan rule as the public constructors (row plans use 64 lanes)
+      const int px = default_plan(Ww + p.sx, true), py = default_plan(Hw + p.sy, false);
+      if (px < 0 || py < 0) return fail(GLSU_ERR_ARG, "no FFT plan for the spot-check window");
       int rc = wff_plan_init(wp, Ww, Hw, p.sigx, p.wxl, p.wxh, p.wxi, p.sigy, p.wyl, p.wyh, p.wyi,
-                             -1.0, -1, -1, -1, -1, err);
+                             -1.0, px, py, -1, -1, err);
       if (rc) return fail(GLSU_ERR_DEVICE, "spot-check plan: %s", err.c_str());
       it = h->wins.emplace(key, wp).first;
     }

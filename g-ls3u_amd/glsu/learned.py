@@ -137,11 +137,15 @@ class UNet:
 
 
 # ------------------------------------------------------------------ data
-def make_pairs(nseq=8, nframes=25, size=1024, seed0=900000, params=None, log=None):
-    """(x, u) pairs from the plain method on synthetic sequences with held-out
-    seeds: x = raw DPRA field z of each frame, u = unit vector of WFF2(z).
-    Per sequence: peaks amplitude in U[2, 5] (+0.1 t), noise std in U[5, 25].
-    Returns two complex64 CUDA tensors (nseq*nframes, size, size)."""
+def make_pairs(nseq=8, nframes=25, size=1024, seed0=900000, params=None, log=None, net=None,
+               keep_every=1):
+    """(x, u) pairs on synthetic sequences with held-out seeds: x = raw DPRA
+    field z of a frame, u = unit vector of WFF2(z).  The chain that produces
+    the reference phase is the plain method, or, with `net`, the learned one
+    (so that the pairs cover the states the learned chain visits; the labels
+    are WFF2 either way).  Every `keep_every`-th frame is kept.  Per
+    sequence: peaks amplitude in U[2, 5] (+0.1 t), noise std in U[5, 25].
+    Returns two complex64 CUDA tensors (frames, size, size)."""
     from .dpra import DPRA, gen_frames
     from .wft2 import WFT2
     params = params if params is not None else wff_params()
@@ -153,14 +157,20 @@ def make_pairs(nseq=8, nframes=25, size=1024, seed0=900000, params=None, log=Non
         fr = gen_frames(size, size, nframes, seed0 + s, amp0=amp0, noise=noise)
         d = DPRA(synth.phase(size, size, amp0, 0.05), params=params)
         for t in range(nframes):
-            d.dpra_per_frame(fr[t])
+            if net is None:
+                d.dpra_per_frame(fr[t])
+            else:
+                d.frame_learned(fr[t], net, tol=-1.0)
             d.update_ref_phi()
+            if t % keep_every:
+                continue
             z = d.last_z()
             F = w(z)
             xs.append(z)
             us.append(F / F.abs().clamp_min(1e-30))
         if log:
-            log(f"pairs: sequence {s + 1}/{nseq} (amp0 {amp0:.2f}, noise {noise:.1f})")
+            log(f"pairs: sequence {s + 1}/{nseq} (amp0 {amp0:.2f}, noise {noise:.1f}"
+                f"{', learned chain' if net is not None else ''})")
     return torch.stack(xs), torch.stack(us)
 
 
@@ -175,7 +185,8 @@ def random_crops(x, u, batch, crop, gen):
     return bx.contiguous(), bu.contiguous()
 
 
-def train(net, x, u, crops=20000, batch=16, crop=256, lr=1e-3, seed=0, log=None, log_every=100):
+def train(net, x, u, crops=20000, batch=16, crop=256, lr=1e-3, seed=0, log=None, log_every=100,
+          warmup=50):
     """Adam on random crops of the pairs until `crops` crops have been seen.
     Returns the list of per-step losses."""
     gen = torch.Generator().manual_seed(seed)
@@ -183,8 +194,9 @@ def train(net, x, u, crops=20000, batch=16, crop=256, lr=1e-3, seed=0, log=None,
     losses = []
     for k in range(steps):
         bx, bu = random_crops(x, u, batch, crop, gen)
-        # cosine learning-rate decay to 5 % of lr
-        lr_k = lr * (0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * k / max(steps - 1, 1))))
+        # linear warm-up, then cosine decay to 5 % of lr
+        lr_k = lr * min(1.0, (k + 1) / max(warmup, 1)) * (
+            0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * k / max(steps - 1, 1))))
         losses.append(net.loss_grad(bx, bu))
         net.adam_step(lr_k)
         if log and (k % log_every == 0 or k == steps - 1):
@@ -199,3 +211,55 @@ def phase_rms(a, b):
     """RMS of the wrapped phase difference of two complex tensors (radians)."""
     d = torch.angle(a * b.conj())
     return float(torch.sqrt(torch.mean(d * d)))
+
+
+def main(argv=None):
+    """python -m glsu.learned --out PATH [--crops N ...]: generate training
+    pairs with the plain method on held-out seeds and train the network."""
+    import argparse
+    import sys
+    import time
+    ap = argparse.ArgumentParser(prog="glsu.learned")
+    ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--crops", type=int, default=20000)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--crop", type=int, default=256)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--nseq", type=int, default=16)
+    ap.add_argument("--nframes", type=int, default=25)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--init", default=None, help="start from these weights")
+    ap.add_argument("--learned-chain", type=int, default=0,
+                    help="extra sequences whose reference phase follows the learned chain "
+                         "of the --init network")
+    ap.add_argument("--keep-every", type=int, default=1)
+    a = ap.parse_args(argv)
+
+    def log(m):
+        print(m, flush=True)
+    t0 = time.perf_counter()
+    net = UNet.load(a.init) if a.init else UNet(4, 32, seed=a.seed)
+    x, u = make_pairs(a.nseq, a.nframes, a.size, log=log, keep_every=a.keep_every)
+    if a.learned_chain:
+        x2, u2 = make_pairs(a.learned_chain, a.nframes, a.size, seed0=950000, log=log, net=net,
+                            keep_every=a.keep_every)
+        x, u = torch.cat([x, x2]), torch.cat([u, u2])
+        del x2, u2
+    torch.cuda.synchronize()
+    log(f"pairs: {tuple(x.shape)} in {time.perf_counter() - t0:.1f} s")
+    t1 = time.perf_counter()
+    losses = train(net, x, u, a.crops, a.batch, a.crop, a.lr, a.seed, log=log)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t1
+    log(f"trained on {a.crops} crops of {a.crop}^2 in {dt:.1f} s "
+        f"({a.crops * a.crop * a.crop / dt / 1e6:.1f} Mpx/s); final loss {np.mean(losses[-20:]):.5f}")
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    net.save(a.out)
+    np.save(os.path.splitext(a.out)[0] + "_losses.npy", np.asarray(losses, np.float32))
+    log(f"saved {a.out}")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

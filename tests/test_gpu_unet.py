@@ -125,6 +125,11 @@ def test_learned_frame_fallback_and_net_paths():
         say(f"[learned] tol 0 frame {t}: residual {res:.3e} fallback {fb}")
         assert fb and res > 0
     assert np.array_equal(np.stack(outs), ref)
+    # spot checks at the corners: the window is clipped by the image border
+    for tile in [(0, 0), (W - 64, H - 64), (0, H - 64)]:
+        res, fb = glsu.DPRA(phi0).frame_learned(frames[0], net, tol=1e9, tile=tile)
+        say(f"[learned] corner tile {tile}: residual {res:.3e}")
+        assert 0 < res < 10 and not fb
     # untrained net = identity (out = z): no check -> the update uses angle(z)
     d2 = glsu.DPRA(phi0)
     res, fb = d2.frame_learned(frames[0], net, tol=-1.0)

@@ -945,7 +945,7 @@ extern "C" int glsu_dpra_batch_create(glsu_dpra_batch** out, const glsu_seq_desc
       sg.row0 = row; sg.col0 = blk; sg.tile0 = tile;
       sg.inv_n = (float)(1.0 / ((double)im.W * im.H));
       row += im.H;
-      blk += (p.nwx * im.W + ipb - 1) / ipb;
+      blk += p.nwx * ((im.W + ipb - 1) / ipb);
       tile += dpra_ls_tiles(im.W, im.H);
     }
     BatchClass& b = c.bc;

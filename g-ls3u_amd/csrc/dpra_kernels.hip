@@ -70,12 +70,15 @@ __device__ __forceinline__ void sincos_half(float x, float* s, float* c) {
   *c = fmaf(-x2, pc, 1.f);                     // 1 - x^2 (1/2! - x^2/4! + ...)
 }
 
-// 64 x 8 pixel tiles (one pixel per thread, 256-byte rows), walked row-major
-// and XCD-contiguous so that halo rows shared by neighbouring tiles are
-// fetched from HBM once per L2.
-constexpr int DTX = 64, DTY = 8;
+// 64 x 8 pixel tiles (256-byte rows), one pixel per thread (DRPT rows per
+// thread is kept as a parameter: the per-pixel body branches, so several
+// pixels per thread serialise their long dependency chains and measured
+// slower), walked row-major and XCD-contiguous so that halo rows shared by
+// neighbouring tiles are fetched from HBM once per L2.
+constexpr int DTX = 64, DTY = 8, DRPT = 1;
+constexpr int DTHREADS = DTX * DTY / DRPT;
 
-// One 64 x 8 tile (bx, by) of one image; all threads of the block take part.
+// One 64 x 16 tile (bx, by) of one image; all threads of the block take part.
 template <typename InT>
 __device__ __forceinline__ void dpra_ls_body(const InT* __restrict__ I,
                                              const float* __restrict__ phr,
@@ -84,9 +87,9 @@ __device__ __forceinline__ void dpra_ls_body(const InT* __restrict__ I,
   __shared__ float sI[DTY + 2][DTX + 2];
   __shared__ float sS[DTY + 2][DTX + 2];  // sin(phi/2)
   __shared__ float sC[DTY + 2][DTX + 2];  // cos(phi/2)
-  const int tx = threadIdx.x % DTX, ty = threadIdx.x / DTX;
+  const int tx = threadIdx.x % DTX, ty0 = (threadIdx.x / DTX) * DRPT;
   const int x0 = bx * DTX - 1, y0 = by * DTY - 1;
-  for (int i = threadIdx.x; i < (DTX + 2) * (DTY + 2); i += DTX * DTY) {
+  for (int i = threadIdx.x; i < (DTX + 2) * (DTY + 2); i += DTHREADS) {
     const int ly = i / (DTX + 2), lx = i % (DTX + 2);
     const int gx = x0 + lx, gy = y0 + ly;
     float vi = 0.f, hs = 0.f, hc = 1.f;
@@ -99,8 +102,12 @@ __device__ __forceinline__ void dpra_ls_body(const InT* __restrict__ I,
     sC[ly][lx] = hc;
   }
   __syncthreads();
-  const int x = bx * DTX + tx, y = by * DTY + ty;
-  if (x >= W || y >= H) return;
+  const int x = bx * DTX + tx;
+  if (x >= W) return;
+#pragma unroll
+  for (int r = 0; r < DRPT; ++r) {
+  const int ty = ty0 + r, y = by * DTY + ty;
+  if (y >= H) continue;
 
   const float hsp = sS[ty + 1][tx + 1], hcp = sC[ty + 1][tx + 1];
   float Se = 0.f, See = 0.f, Sss = 0.f, SI = 0.f, SIe = 0.f;
@@ -186,10 +193,11 @@ __device__ __forceinline__ void dpra_ls_body(const InT* __restrict__ I,
     out = make_float2(ur * cp + ui * sp, ui * cp - ur * sp);  // * e^{-j phi_p}
   }
   z[(size_t)y * W + x] = out;
+  }  // rows of this thread
 }
 
 template <typename InT>
-__global__ __launch_bounds__(DTX* DTY) void dpra_ls_kernel(const InT* __restrict__ I,
+__global__ __launch_bounds__(DTHREADS) void dpra_ls_kernel(const InT* __restrict__ I,
                                                            const float* __restrict__ phr,
                                                            float2* __restrict__ z, int W, int H,
                                                            int pitch) {
@@ -202,7 +210,7 @@ __global__ __launch_bounds__(DTX* DTY) void dpra_ls_kernel(const InT* __restrict
 // table; frame t of image s is frames_s + t*W*H, its reference phase the
 // buffer that the swap parity of t names.
 template <typename InT>
-__global__ __launch_bounds__(DTX* DTY) void dpra_ls_batch_kernel(const BatchSeg* __restrict__ tab,
+__global__ __launch_bounds__(DTHREADS) void dpra_ls_batch_kernel(const BatchSeg* __restrict__ tab,
                                                                  int nseg, int ntiles,
                                                                  const int* __restrict__ frame) {
   const int t = *frame;
@@ -228,15 +236,15 @@ hipError_t launch_dpra_ls_batch(const BatchSeg* tab, int nseg, int ntiles, int g
                                 const int* frame, hipStream_t s) {
   switch (dtype) {
     case IN_U8:
-      hipLaunchKernelGGL(dpra_ls_batch_kernel<uint8_t>, dim3(grid), dim3(DTX * DTY), 0, s, tab,
+      hipLaunchKernelGGL(dpra_ls_batch_kernel<uint8_t>, dim3(grid), dim3(DTHREADS), 0, s, tab,
                          nseg, ntiles, frame);
       break;
     case IN_F32:
-      hipLaunchKernelGGL(dpra_ls_batch_kernel<float>, dim3(grid), dim3(DTX * DTY), 0, s, tab,
+      hipLaunchKernelGGL(dpra_ls_batch_kernel<float>, dim3(grid), dim3(DTHREADS), 0, s, tab,
                          nseg, ntiles, frame);
       break;
     case IN_F64:
-      hipLaunchKernelGGL(dpra_ls_batch_kernel<double>, dim3(grid), dim3(DTX * DTY), 0, s, tab,
+      hipLaunchKernelGGL(dpra_ls_batch_kernel<double>, dim3(grid), dim3(DTHREADS), 0, s, tab,
                          nseg, ntiles, frame);
       break;
     default:
@@ -297,7 +305,7 @@ int batch_grid_ls(int ntiles) {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, dpra_ls_batch_kernel<float>, DTX * DTY,
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, dpra_ls_batch_kernel<float>, DTHREADS,
                                                     0) != hipSuccess)
     return -1;
   const int cap = std::max(8, (cus * std::max(per, 1)) / 8 * 8);
@@ -308,7 +316,7 @@ int dpra_ls_tiles(int W, int H) { return ((W + DTX - 1) / DTX) * ((H + DTY - 1) 
 
 hipError_t launch_dpra_ls(const void* frame, int dtype, const float* phiref, float2* z, int W,
                           int H, int pitch, hipStream_t s) {
-  dim3 grid(((W + DTX - 1) / DTX) * ((H + DTY - 1) / DTY)), block(DTX * DTY);
+  dim3 grid(((W + DTX - 1) / DTX) * ((H + DTY - 1) / DTY)), block(DTHREADS);
   switch (dtype) {
     case IN_U8:
       hipLaunchKernelGGL(dpra_ls_kernel<uint8_t>, grid, block, 0, s, (const uint8_t*)frame,

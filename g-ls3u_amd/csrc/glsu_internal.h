@@ -134,7 +134,7 @@ struct BatchClass {
   const BatchSeg* tab;       // device table of this class's images (contiguous)
   int nseg;
   int nrows;                 // sum of H
-  int ncolblk;               // sum over images of ceil(nwx*W / items per block)
+  int ncolblk;               // sum over images of nwx * ceil(W / items per block)
   int planx, plany, cols_waves, rows_waves;
   int nwx, nwy;
   float scale;

@@ -10,8 +10,7 @@ hipError_t launch_cols(const WffPlan& p, hipStream_t s) {
 #define BODY                                                                                    \
   {                                                                                             \
     const int per = items_per_block<P>(waves);                                                  \
-    const int items = p.nwx * p.W;                                                              \
-    const int grid = (items + per - 1) / per;                                                   \
+    const int grid = p.nwx * ((p.W + per - 1) / per);                                           \
     switch (waves) {                                                                            \
       case 1:                                                                                   \
         hipLaunchKernelGGL((wff_cols_kernel<P, 1>), dim3(grid), dim3(64), smem, s, p.d_A,      \

@@ -144,12 +144,19 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
       v[j] = cmul(ah[j], hy[P::koff(j)]);
     });
     fft_inv<P>(v, buf, ctx);
+    bool live = false;
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       const int y = lane + T * j;
       const float m2 = v[j].x * v[j].x + v[j].y * v[j].y;
-      if (!(y < H && m2 >= thr2)) v[j] = make_float2(0.f, 0.f);
+      const bool keep = y < H && m2 >= thr2;
+      if (!keep) v[j] = make_float2(0.f, 0.f);
+      live |= keep;
     }
+    // a column whose thresholded signal is zero everywhere adds exactly
+    // nothing: skip its transform when no slot of the wavefront has a
+    // surviving sample (wave-uniform branch)
+    if (__builtin_amdgcn_ballot_w64(live) == 0) continue;
     fft_fwd<P>(v, buf, ctx);
     // re-read the window spectrum instead of keeping 2E VGPRs live across the
     // two transforms: the opaque copy of the lane offset defeats load CSE
@@ -184,9 +191,15 @@ __global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
   __syncthreads();
   const int slot = threadIdx.x / T;
   float2* buf = smem + E * T + slot * P::LDS;
-  const int item = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / T) + slot;
-  if (item >= nwx * W) return;
-  cols_body<P>(A, D, Hy, tw_g, *thr_p, W, H, nwx, nwy, plane, item, threadIdx.x % T, ttab, buf);
+  // block b: plane k = b % nwx, columns [(b / nwx) * ipb, +ipb).  Planes cycle
+  // fastest so that every XCD's contiguous block range holds an even mix of
+  // wx: columns whose thresholded signal is zero (skipped transforms) cluster
+  // by wx, and the kernel lasts as long as its busiest XCD.
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int x = (b / nwx) * (blockDim.x / T) + slot;
+  if (x >= W) return;
+  cols_body<P>(A, D, Hy, tw_g, *thr_p, W, H, nwx, nwy, plane, (b % nwx) * W + x,
+               threadIdx.x % T, ttab, buf);
 }
 
 // ---------------------------------------------------------- row inverse
@@ -334,9 +347,10 @@ __global__ __launch_bounds__(64 * WAVES) void wff_cols_batch_kernel(
   for (int v = blockIdx.x; v < nblk; v += gridDim.x) {
     const int b = xcd_remap(v, nblk);
     const BatchSeg& s = tab[seg_find<&BatchSeg::col0>(tab, nseg, b)];
-    const int item = (b - s.col0) * (blockDim.x / T) + slot;
-    if (item < nwx * s.W)
-      cols_body<P>(s.A, s.D, Hy, tw_g, *s.thr, s.W, s.H, nwx, nwy, s.plane, item,
+    const int lb = b - s.col0;  // plane lb % nwx, columns [(lb / nwx) * ipb, +ipb)
+    const int x = (lb / nwx) * (blockDim.x / T) + slot;
+    if (x < s.W)
+      cols_body<P>(s.A, s.D, Hy, tw_g, *s.thr, s.W, s.H, nwx, nwy, s.plane, (lb % nwx) * s.W + x,
                    opaque_lane<P>(), ttab, buf);
   }
 }

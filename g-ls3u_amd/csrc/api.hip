@@ -569,7 +569,7 @@ static int dpra_run_enqueue(glsu_dpra* h, const void* frames, int nframes, int d
 
 extern "C" int glsu_dpra_run(glsu_dpra* h, const void* frames, int nframes, int dtype,
                              float* dphi_all, double* ms) {
-  if (!h || !frames || nframes < 0) return fail(GLSU_ERR_ARG, "invalid argument");
+  if (!h || nframes < 0 || (!frames && nframes > 0)) return fail(GLSU_ERR_ARG, "invalid argument");
   if (!elem_size(dtype)) return fail(GLSU_ERR_ARG, "unknown frame dtype %d", dtype);
   if (ms) HIP_TRY(hipEventRecord(h->e0, h->s));
   if (int rc = dpra_run_enqueue(h, frames, nframes, dtype, dphi_all, h->s)) return rc;
@@ -585,7 +585,7 @@ extern "C" int glsu_dpra_run(glsu_dpra* h, const void* frames, int nframes, int 
 
 extern "C" int glsu_dpra_run_graph(glsu_dpra* h, const void* frames, int nframes, int dtype,
                                    float* dphi_all, double* ms) {
-  if (!h || !frames || nframes < 0) return fail(GLSU_ERR_ARG, "invalid argument");
+  if (!h || nframes < 0 || (!frames && nframes > 0)) return fail(GLSU_ERR_ARG, "invalid argument");
   if (!elem_size(dtype)) return fail(GLSU_ERR_ARG, "unknown frame dtype %d", dtype);
   GraphKey key{frames, nframes, dtype, dphi_all, h->phiref, h->p.thr_ready, h->fresh};
   auto it = h->graphs.find(key);
@@ -989,7 +989,7 @@ static int swaps_before(int t, int rr) { return t == 0 ? 0 : (t - 1) / rr + 1; }
 extern "C" int glsu_dpra_batch_run(glsu_dpra_batch* h, const void* const* frames, int nframes,
                                    int dtype, float* const* dphi_all, int use_graph,
                                    double* ms) {
-  if (!h || !frames || nframes < 0) return fail(GLSU_ERR_ARG, "invalid argument");
+  if (!h || nframes < 0 || (!frames && nframes > 0)) return fail(GLSU_ERR_ARG, "invalid argument");
   if (!elem_size(dtype)) return fail(GLSU_ERR_ARG, "unknown frame dtype %d", dtype);
   for (int i = 0; i < h->n; ++i)
     if (!frames[i]) return fail(GLSU_ERR_ARG, "sequence %d: null frames", i);

@@ -41,14 +41,20 @@ __device__ __forceinline__ bf16x8 load8(const __bf16* __restrict__ X1, int C1,
 // ------------------------------------------------------------------------
 // 3x3 convolution, implicit GEMM: M = pixels, N = output channels, K = 9*Cin.
 // Workgroup: 4 waves, output tile 4 rows x 64 pixels x NCO channels; wave w
-// owns row w.  Per 32-channel chunk the input tile (6 x 66 pixels) and the
-// chunk's weights (9 taps x NCO) are staged in LDS, then each wave issues
-// 4 x (NCO/16) MFMAs per tap.
+// owns row w.  Per 32-channel chunk the input tile (6 x 66 pixels x 32
+// channels, 25 KB) is staged in LDS; each 16-byte group of 8 channels of LDS
+// pixel p sits at group slot g ^ ((p >> 1) & 3), which keeps the MFMA operand
+// reads (16 consecutive pixels, one group) free of bank conflicts without
+// padding.  The weight operands are read straight from global memory (they
+// are small and shared by every workgroup, so they stay in L1/L2), one tap
+// ahead of their use.  Each wave issues 4 x (NCO/16) MFMAs per tap.
 // MODE 0: y = relu(acc + bias).  MODE 1: y = acc * (mask > 0) (mask may be
 // null: y = acc).  Used for the forward pass (weights [Cout][9][Cin]) and
 // for the input gradient (dY as input, flipped transposed weights).
-constexpr int kPadC = 40;  // LDS pitch per pixel / output channel (bf16): 80 B
 constexpr int kTileW = 64, kTileH = 4;
+constexpr int kInPix = (kTileH + 2) * (kTileW + 2);
+
+__device__ __forceinline__ int swz(int p, int g) { return p * 32 + ((g ^ ((p >> 1) & 3)) << 3); }
 
 template <int NCO, int MODE>
 __global__ __launch_bounds__(256) void conv3_mfma_kernel(
@@ -56,8 +62,12 @@ __global__ __launch_bounds__(256) void conv3_mfma_kernel(
     const __bf16* __restrict__ Wt, const float* __restrict__ bias,
     const __bf16* __restrict__ mask, __bf16* __restrict__ Y, int n, int H, int W, int Cout) {
   constexpr int NB = NCO / 16;
-  __shared__ __attribute__((aligned(16))) __bf16 sIn[(kTileH + 2) * (kTileW + 2) * kPadC];
-  __shared__ __attribute__((aligned(16))) __bf16 sW[9 * NCO * kPadC];
+  // 64-channel tiles read each weight fragment from 4 waves: stage the weights
+  // in LDS, one row of 3 taps at a time; 32-channel tiles read them from
+  // global memory (L1), which leaves LDS for more resident workgroups
+  constexpr bool kWLds = NCO >= 64;
+  __shared__ __attribute__((aligned(16))) __bf16 sIn[kInPix * 32];
+  __shared__ __attribute__((aligned(16))) __bf16 sW[kWLds ? 3 * NCO * 32 : 8];
   const int Cin = C1 + C2;
   const int tilesx = (W + kTileW - 1) / kTileW;
   const int tilesy = H / kTileH;
@@ -75,41 +85,72 @@ __global__ __launch_bounds__(256) void conv3_mfma_kernel(
 #pragma unroll
     for (int j = 0; j < NB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
-    __syncthreads();  // previous chunk fully consumed
-    // input tile: (kTileH + 2) x (kTileW + 2) pixels x 4 groups of 8 channels
-    for (int i = threadIdx.x; i < (kTileH + 2) * (kTileW + 2) * 4; i += 256) {
-      const int g = i & 3, pix = i >> 2;
-      const int ry = pix / (kTileW + 2), rx = pix % (kTileW + 2);
-      *(bf16x8*)(sIn + pix * kPadC + g * 8) =
-          load8(X1, C1, X2, C2, img, y0 + ry - 1, x0 + rx - 1, H, W, ci0 + g * 8);
-    }
-    // weights of this chunk: 9 taps x NCO output channels x 32 channels
-    for (int i = threadIdx.x; i < 9 * NCO * 4; i += 256) {
-      const int g = i & 3, r = i >> 2;
-      const int t = r / NCO, co = r % NCO;
-      bf16x8 v = zero8();
-      if (co0 + co < Cout)
-        v = *(const bf16x8*)(Wt + ((size_t)(co0 + co) * 9 + t) * Cin + ci0 + g * 8);
-      *(bf16x8*)(sW + (t * NCO + co) * kPadC + g * 8) = v;
-    }
-    __syncthreads();
-#pragma unroll 1
-    for (int t = 0; t < 9; ++t) {
-      const int dy = t / 3, dx = t % 3;  // offsets + 1
-      bf16x8 a[4], b[NB];
+  // weight operand of block nb at tap t of chunk ci0: channel co0 + 16 nb + lr,
+  // input channels ci0 + 8 lk .. + 7
+  auto load_b = [&](bf16x8* b, int t, int ci0) {
 #pragma unroll
-      for (int mb = 0; mb < 4; ++mb)
-        a[mb] = *(const bf16x8*)(sIn + ((wave + dy) * (kTileW + 2) + mb * 16 + lr + dx) * kPadC +
-                                 lk * 8);
+    for (int nb = 0; nb < NB; ++nb) {
+      const int co = co0 + nb * 16 + lr;
+      b[nb] = co < Cout ? *(const bf16x8*)(Wt + ((size_t)co * 9 + t) * Cin + ci0 + lk * 8)
+                        : zero8();
+    }
+  };
+  auto tap = [&](int t, const bf16x8* b) {
+    const int dy = t / 3, dx = t % 3;  // offsets + 1
+    bf16x8 a[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+      a[mb] = *(const bf16x8*)(sIn + swz((wave + dy) * (kTileW + 2) + mb * 16 + lr + dx, lk));
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb)
-        b[nb] = *(const bf16x8*)(sW + (t * NCO + nb * 16 + lr) * kPadC + lk * 8);
+        acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb], b[nb], acc[mb][nb], 0, 0, 0);
+  };
+
+  for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
+    bf16x8 b[NB];
+    if (!kWLds) load_b(b, 0, ci0);
+    __syncthreads();  // previous chunk fully consumed
+    for (int i = threadIdx.x; i < kInPix * 4; i += 256) {
+      const int g = i & 3, pix = i >> 2;
+      const int ry = pix / (kTileW + 2), rx = pix % (kTileW + 2);
+      *(bf16x8*)(sIn + swz(pix, g)) =
+          load8(X1, C1, X2, C2, img, y0 + ry - 1, x0 + rx - 1, H, W, ci0 + g * 8);
+    }
+    if constexpr (kWLds) {
+#pragma unroll 1
+      for (int tg = 0; tg < 3; ++tg) {
+        if (tg > 0) __syncthreads();  // previous tap row consumed
+        for (int i = threadIdx.x; i < 3 * NCO * 4; i += 256) {
+          const int g = i & 3, r = i >> 2;
+          const int tl = r / NCO, co = r % NCO;
+          bf16x8 v = zero8();
+          if (co0 + co < Cout)
+            v = *(const bf16x8*)(Wt + ((size_t)(co0 + co) * 9 + 3 * tg + tl) * Cin + ci0 + g * 8);
+          *(bf16x8*)(sW + swz(r, g)) = v;
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int tl = 0; tl < 3; ++tl) {
 #pragma unroll
-      for (int mb = 0; mb < 4; ++mb)
+          for (int nb = 0; nb < NB; ++nb)
+            b[nb] = *(const bf16x8*)(sW + swz(tl * NCO + nb * 16 + lr, lk));
+          tap(3 * tg + tl, b);
+        }
+      }
+    } else {
+      __syncthreads();
+#pragma unroll 1
+      for (int t = 0; t < 9; ++t) {
+        bf16x8 bn[NB];
+        if (t < 8) load_b(bn, t + 1, ci0);
+        tap(t, b);
+        if (t < 8) {
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb)
-          acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb], b[nb], acc[mb][nb], 0, 0, 0);
+          for (int nb = 0; nb < NB; ++nb) b[nb] = bn[nb];
+        }
+      }
     }
   }
   // epilogue: lane holds rows m = 4*lk + j (pixels) of column lr (channel)

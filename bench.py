@@ -247,16 +247,17 @@ def _opt(n):
 def _traffic(W, H, T, tuning):
     """HBM bytes per launch of the column kernel from the committed rocprofv3
     --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration), when that summary was taken on this workload and tuning."""
+    calibration), when it holds an entry for this size and tuning."""
     path = os.path.join(ROOT, "profiles", f"wff_cols_{W}x{H}_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("tuning") != tuning or t.get("size") != [H, W]:
-        return None
-    return {"bytes": int(t["hbm_bytes_per_launch"]), "source": os.path.relpath(path, ROOT)}
+    for e in t.get("entries", []):
+        if e.get("tuning") == tuning and e.get("size") == [H, W]:
+            return {"bytes": int(e["hbm_bytes_per_launch"]), "source": os.path.relpath(path, ROOT)}
+    return None
 
 
 def main():

@@ -376,8 +376,9 @@ struct glsu_dpra {
   float2 *d_zwin = nullptr, *d_fwin = nullptr;  // spot-check window in / filtered
   size_t win_cap = 0;
   std::map<std::pair<int, int>, WffPlan> wins;  // WFF2 plans by window size
-  float* d_res = nullptr;
-  float* h_res = nullptr;                       // pinned
+  int* d_flags = nullptr;                       // [fallback, !fallback] of the last frame
+  float* d_stats = nullptr;                     // frames, fallbacks, sum rms, max rms, last rms
+  float* h_res = nullptr;                       // pinned: last rms, fallback flag
 };
 
 static int dpra_alloc(glsu_dpra* h) {
@@ -407,7 +408,8 @@ static void dpra_free(glsu_dpra* h) {
   hipFree(h->phiref); hipFree(h->phicurr);
   hipFree(h->dref_hi); hipFree(h->dref_lo); hipFree(h->dphi_hi); hipFree(h->dphi_lo);
   hipFree(h->z); hipFree(h->d_stage); hipFree(h->d_dphi_stage);
-  hipFree(h->d_net); hipFree(h->d_zwin); hipFree(h->d_fwin); hipFree(h->d_res);
+  hipFree(h->d_net); hipFree(h->d_zwin); hipFree(h->d_fwin); hipFree(h->d_flags);
+  hipFree(h->d_stats);
   if (h->h_res) hipHostFree(h->h_res);
   for (auto& kv : h->wins) wff_plan_free(kv.second);
   wff_plan_free(h->p);
@@ -1107,72 +1109,98 @@ extern "C" int glsu_dpra_frame_learned(glsu_dpra* h, glsu_unet* net, const void*
   WffPlan& p = h->p;
   const size_t n = (size_t)W * H;
   if (!h->d_net) HIP_TRY(hipMalloc((void**)&h->d_net, n * sizeof(float2)));
+  if (!h->d_flags) {
+    HIP_TRY(hipMalloc((void**)&h->d_flags, 2 * sizeof(int)));
+    HIP_TRY(hipMalloc((void**)&h->d_stats, 5 * sizeof(float)));
+    HIP_TRY(hipMemset(h->d_stats, 0, 5 * sizeof(float)));
+    HIP_TRY(hipHostMalloc((void**)&h->h_res, 2 * sizeof(float)));
+  }
   HIP_TRY(launch_dpra_ls(frame, dtype, h->phiref, h->z, W, H, W, h->s));
   if (!p.thr_ready) {
     HIP_TRY(launch_thr(p, h->z, h->s));
     p.thr_ready = true;
   }
   if (int rc = unet_forward_on(net, h->z, h->d_net, H, W, h->s)) return rc;
-  double res = -1.0;
-  if (tol >= 0) {
-    const int tx = std::min(std::max(tile_x, 0), W - 64), ty = std::min(std::max(tile_y, 0), H - 64);
-    const int x0 = std::max(tx - 2 * p.sx, 0), x1 = std::min(tx + 64 + 2 * p.sx, W);
-    const int y0 = std::max(ty - 2 * p.sy, 0), y1 = std::min(ty + 64 + 2 * p.sy, H);
-    const int Ww = x1 - x0, Hw = y1 - y0;
-    auto key = std::make_pair(Ww, Hw);
-    auto it = h->wins.find(key);
-    if (it == h->wins.end()) {
-      WffPlan wp;
-      std::string err;
-      // same plan rule as the public constructors (row plans use 64 lanes)
-      const int px = default_plan(Ww + p.sx, true), py = default_plan(Hw + p.sy, false);
-      if (px < 0 || py < 0) return fail(GLSU_ERR_ARG, "no FFT plan for the spot-check window");
-      int rc = wff_plan_init(wp, Ww, Hw, p.sigx, p.wxl, p.wxh, p.wxi, p.sigy, p.wyl, p.wyh, p.wyi,
-                             -1.0, px, py, -1, -1, err);
-      if (rc) return fail(GLSU_ERR_DEVICE, "spot-check plan: %s", err.c_str());
-      it = h->wins.emplace(key, wp).first;
-    }
-    WffPlan& wp = it->second;
-    if ((size_t)Ww * Hw > h->win_cap) {
-      hipFree(h->d_zwin);
-      hipFree(h->d_fwin);
-      h->d_zwin = h->d_fwin = nullptr;
-      h->win_cap = (size_t)Ww * Hw;
-      HIP_TRY(hipMalloc((void**)&h->d_zwin, h->win_cap * sizeof(float2)));
-      HIP_TRY(hipMalloc((void**)&h->d_fwin, h->win_cap * sizeof(float2)));
-    }
-    if (!h->d_res) {
-      HIP_TRY(hipMalloc((void**)&h->d_res, sizeof(float)));
-      HIP_TRY(hipHostMalloc((void**)&h->h_res, sizeof(float)));
-    }
-    // the window uses the full image's threshold
-    HIP_TRY(hipMemcpyAsync(wp.d_thr, p.d_thr, sizeof(float), hipMemcpyDeviceToDevice, h->s));
-    wp.thr_ready = true;
-    HIP_TRY(hipMemcpy2DAsync(h->d_zwin, Ww * sizeof(float2), h->z + (size_t)y0 * W + x0,
-                             W * sizeof(float2), Ww * sizeof(float2), Hw,
-                             hipMemcpyDeviceToDevice, h->s));
-    HIP_TRY(launch_rows_fwd(wp, h->d_zwin, h->s));
-    HIP_TRY(launch_cols(wp, h->s));
-    HIP_TRY(launch_rows_inv_filtered(wp, h->d_fwin, h->s));
-    HIP_TRY(launch_tile_residual(h->d_net, W, tx, ty, h->d_fwin, Ww, tx - x0, ty - y0, h->d_res,
-                                 h->s));
-    HIP_TRY(hipMemcpyAsync(h->h_res, h->d_res, sizeof(float), hipMemcpyDeviceToHost, h->s));
-    HIP_TRY(hipStreamSynchronize(h->s));
-    res = std::sqrt((double)*h->h_res / 4096.0);
-  }
-  const bool fb = tol >= 0 && res > tol;
   UpdateArgs u{h->phiref, h->dref_hi, h->dref_lo, h->phicurr, h->dphi_hi, h->dphi_lo, d_dphi,
                nullptr};
-  if (fb) {
-    HIP_TRY(launch_rows_fwd(p, h->z, h->s));
-    HIP_TRY(launch_cols(p, h->s));
-    HIP_TRY(launch_rows_inv_update(p, u, h->s));
-  } else {
-    HIP_TRY(launch_net_update(h->d_net, u, n, h->s));
+  if (tol < 0) {
+    HIP_TRY(launch_net_update(h->d_net, u, n, h->s, nullptr));
+    h->fresh = true;
+    if (residual) *residual = -1.0;
+    if (fallback) *fallback = 0;
+    return GLSU_OK;
   }
+  const int tx = std::min(std::max(tile_x, 0), W - 64), ty = std::min(std::max(tile_y, 0), H - 64);
+  const int x0 = std::max(tx - 2 * p.sx, 0), x1 = std::min(tx + 64 + 2 * p.sx, W);
+  const int y0 = std::max(ty - 2 * p.sy, 0), y1 = std::min(ty + 64 + 2 * p.sy, H);
+  const int Ww = x1 - x0, Hw = y1 - y0;
+  auto key = std::make_pair(Ww, Hw);
+  auto it = h->wins.find(key);
+  if (it == h->wins.end()) {
+    WffPlan wp;
+    std::string err;
+    // same plan rule as the public constructors (row plans use 64 lanes)
+    const int px = default_plan(Ww + p.sx, true), py = default_plan(Hw + p.sy, false);
+    if (px < 0 || py < 0) return fail(GLSU_ERR_ARG, "no FFT plan for the spot-check window");
+    int rc = wff_plan_init(wp, Ww, Hw, p.sigx, p.wxl, p.wxh, p.wxi, p.sigy, p.wyl, p.wyh, p.wyi,
+                           -1.0, px, py, -1, -1, err);
+    if (rc) return fail(GLSU_ERR_DEVICE, "spot-check plan: %s", err.c_str());
+    it = h->wins.emplace(key, wp).first;
+  }
+  WffPlan& wp = it->second;
+  if ((size_t)Ww * Hw > h->win_cap) {
+    HIP_TRY(hipStreamSynchronize(h->s));
+    hipFree(h->d_zwin);
+    hipFree(h->d_fwin);
+    h->d_zwin = h->d_fwin = nullptr;
+    h->win_cap = (size_t)Ww * Hw;
+    HIP_TRY(hipMalloc((void**)&h->d_zwin, h->win_cap * sizeof(float2)));
+    HIP_TRY(hipMalloc((void**)&h->d_fwin, h->win_cap * sizeof(float2)));
+  }
+  // the window uses the full image's threshold
+  HIP_TRY(hipMemcpyAsync(wp.d_thr, p.d_thr, sizeof(float), hipMemcpyDeviceToDevice, h->s));
+  wp.thr_ready = true;
+  HIP_TRY(hipMemcpy2DAsync(h->d_zwin, Ww * sizeof(float2), h->z + (size_t)y0 * W + x0,
+                           W * sizeof(float2), Ww * sizeof(float2), Hw, hipMemcpyDeviceToDevice,
+                           h->s));
+  HIP_TRY(launch_rows_fwd(wp, h->d_zwin, h->s));
+  HIP_TRY(launch_cols(wp, h->s));
+  HIP_TRY(launch_rows_inv_filtered(wp, h->d_fwin, h->s));
+  HIP_TRY(launch_tile_residual(h->d_net, W, tx, ty, h->d_fwin, Ww, tx - x0, ty - y0, (float)tol,
+                               h->d_flags, h->d_flags + 1, h->d_stats, h->s));
+  // both paths are enqueued; the device flag lets exactly one of them run, so
+  // the host never waits for the decision
+  int* not_fb = h->d_flags + 1;
+  HIP_TRY(launch_rows_fwd(p, h->z, h->s, not_fb));
+  HIP_TRY(launch_cols(p, h->s, not_fb));
+  HIP_TRY(launch_rows_inv_update(p, u, h->s, not_fb));
+  HIP_TRY(launch_net_update(h->d_net, u, n, h->s, h->d_flags));
   h->fresh = true;
-  if (residual) *residual = res;
-  if (fallback) *fallback = fb ? 1 : 0;
+  if (residual || fallback) {
+    HIP_TRY(hipMemcpyAsync(h->h_res, h->d_stats + 4, sizeof(float), hipMemcpyDeviceToHost, h->s));
+    HIP_TRY(hipMemcpyAsync(h->h_res + 1, h->d_flags, sizeof(int), hipMemcpyDeviceToHost, h->s));
+    HIP_TRY(hipStreamSynchronize(h->s));
+    int fb = 0;
+    std::memcpy(&fb, h->h_res + 1, sizeof(int));
+    if (residual) *residual = h->h_res[0];
+    if (fallback) *fallback = fb;
+  }
+  return GLSU_OK;
+}
+
+extern "C" int glsu_dpra_learned_stats(glsu_dpra* h, int* frames, int* fallbacks,
+                                       double* mean_rms, double* max_rms, int reset) {
+  if (!h) return fail(GLSU_ERR_ARG, "null handle");
+  float st[5] = {0, 0, 0, 0, 0};
+  if (h->d_stats) {
+    HIP_TRY(hipStreamSynchronize(h->s));
+    HIP_TRY(hipMemcpy(st, h->d_stats, sizeof st, hipMemcpyDeviceToHost));
+    if (reset) HIP_TRY(hipMemset(h->d_stats, 0, sizeof st));
+  }
+  if (frames) *frames = (int)st[0];
+  if (fallbacks) *fallbacks = (int)st[1];
+  if (mean_rms) *mean_rms = st[0] > 0 ? st[2] / st[0] : 0.0;
+  if (max_rms) *max_rms = st[3];
   return GLSU_OK;
 }
 

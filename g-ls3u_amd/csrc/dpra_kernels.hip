@@ -256,25 +256,28 @@ hipError_t launch_dpra_ls_batch(const BatchSeg* tab, int nseg, int ntiles, int g
 // ---------------------------------------------------------------------------
 // Learned path: phase update from the network output, and the tile residual.
 __global__ __launch_bounds__(256) void net_update_kernel(const float2* __restrict__ out,
-                                                         UpdateArgs u, size_t n) {
+                                                         UpdateArgs u, size_t n,
+                                                         const int* __restrict__ skip) {
+  if (skip && *skip) return;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float2 o = out[i];
   phase_update(u, i, atan2f(o.y, o.x));
 }
 
-hipError_t launch_net_update(const float2* out, const UpdateArgs& u, size_t n, hipStream_t s) {
+hipError_t launch_net_update(const float2* out, const UpdateArgs& u, size_t n, hipStream_t s,
+                             const int* skip) {
   hipLaunchKernelGGL(net_update_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out,
-                     u, n);
+                     u, n, skip);
   return hipGetLastError();
 }
 
-// sum over a 64 x 64 tile of wrap(angle(net) - angle(fwin))^2, added to *acc
-__global__ __launch_bounds__(256) void tile_residual_kernel(const float2* __restrict__ net, int W,
-                                                            int tx, int ty,
-                                                            const float2* __restrict__ fwin,
-                                                            int Ww, int ox, int oy,
-                                                            float* __restrict__ acc) {
+// RMS over a 64 x 64 tile of wrap(angle(net) - angle(fwin)); sets the
+// fallback flag (and its complement) and accumulates the statistics
+__global__ __launch_bounds__(256) void tile_residual_kernel(
+    const float2* __restrict__ net, int W, int tx, int ty, const float2* __restrict__ fwin, int Ww,
+    int ox, int oy, float tol, int* __restrict__ flag, int* __restrict__ not_flag,
+    float* __restrict__ stats) {
   __shared__ float red[256];
   float s = 0.f;
   for (int i = threadIdx.x; i < 64 * 64; i += 256) {
@@ -291,13 +294,24 @@ __global__ __launch_bounds__(256) void tile_residual_kernel(const float2* __rest
     if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *acc = red[0];
+  if (threadIdx.x == 0) {
+    const float rms = sqrtf(red[0] / 4096.f);
+    const int fb = rms > tol ? 1 : 0;
+    *flag = fb;
+    *not_flag = 1 - fb;
+    stats[0] += 1.f;
+    stats[1] += (float)fb;
+    stats[2] += rms;
+    stats[3] = fmaxf(stats[3], rms);
+    stats[4] = rms;  // last frame's value
+  }
 }
 
 hipError_t launch_tile_residual(const float2* net, int W, int tx, int ty, const float2* fwin,
-                                int Ww, int ox, int oy, float* acc, hipStream_t s) {
+                                int Ww, int ox, int oy, float tol, int* flag, int* not_flag,
+                                float* stats, hipStream_t s) {
   hipLaunchKernelGGL(tile_residual_kernel, dim3(1), dim3(256), 0, s, net, W, tx, ty, fwin, Ww, ox,
-                     oy, acc);
+                     oy, tol, flag, not_flag, stats);
   return hipGetLastError();
 }
 

@@ -62,8 +62,10 @@ void wff_plan_free(WffPlan& p);
 
 // Kernel launchers (all asynchronous on `s`).
 hipError_t launch_thr(const WffPlan& p, const float2* f, hipStream_t s);
-hipError_t launch_rows_fwd(const WffPlan& p, const float2* f, hipStream_t s);
-hipError_t launch_cols(const WffPlan& p, hipStream_t s);
+// skip (optional device flag): when *skip != 0 the kernel exits at once
+hipError_t launch_rows_fwd(const WffPlan& p, const float2* f, hipStream_t s,
+                           const int* skip = nullptr);
+hipError_t launch_cols(const WffPlan& p, hipStream_t s, const int* skip = nullptr);
 // mode 0: out = filtered complex field (H*W float2)
 hipError_t launch_rows_inv_filtered(const WffPlan& p, float2* out, hipStream_t s);
 // mode 1: DPRA phase update fused into the inverse row pass
@@ -77,7 +79,8 @@ struct UpdateArgs {
   float* dphi_out;   // may be null
   float2* filtered;  // may be null
 };
-hipError_t launch_rows_inv_update(const WffPlan& p, const UpdateArgs& u, hipStream_t s);
+hipError_t launch_rows_inv_update(const WffPlan& p, const UpdateArgs& u, hipStream_t s,
+                                  const int* skip = nullptr);
 
 // DPRA phase update of pixel id from the filtered phase t = angle(F)
 // (DPRA/dpra_cpu.cpp:244-251): phicurr = wrap(phiref + t) into (-pi, pi]
@@ -101,9 +104,14 @@ __device__ __forceinline__ void phase_update(const UpdateArgs& u, size_t id, flo
 }
 // learned path: update from the network output (t = angle(out)); residual
 // of the network's phase against a WFF2 window on a 64 x 64 tile
-hipError_t launch_net_update(const float2* out, const UpdateArgs& u, size_t n, hipStream_t s);
+// skip: device flag, the update is skipped when *skip != 0 (the frame fell back)
+hipError_t launch_net_update(const float2* out, const UpdateArgs& u, size_t n, hipStream_t s,
+                             const int* skip);
+// flag = rms > tol, not_flag = !flag; stats[0] += 1 (frames), stats[1] += flag,
+// stats[2] += rms, stats[3] = max(stats[3], rms), stats[4] = rms (single-thread update)
 hipError_t launch_tile_residual(const float2* net, int W, int tx, int ty, const float2* fwin,
-                                int Ww, int ox, int oy, float* acc, hipStream_t s);
+                                int Ww, int ox, int oy, float tol, int* flag, int* not_flag,
+                                float* stats, hipStream_t s);
 
 // ---- batched launches over many images --------------------------------
 // One image of a batched launch (device-resident table entry).  row0 / col0 /

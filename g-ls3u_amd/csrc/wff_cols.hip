@@ -3,7 +3,7 @@
 
 namespace glsu {
 
-hipError_t launch_cols(const WffPlan& p, hipStream_t s) {
+hipError_t launch_cols(const WffPlan& p, hipStream_t s, const int* skip) {
   const int waves = p.cols_waves;
   const int smem = plan_smem_bytes(p.plany, waves, true);
   switch (p.plany) {
@@ -15,17 +15,17 @@ hipError_t launch_cols(const WffPlan& p, hipStream_t s) {
       case 1:                                                                                   \
         hipLaunchKernelGGL((wff_cols_kernel<P, 1>), dim3(grid), dim3(64), smem, s, p.d_A,      \
                            p.d_D, p.d_Hy, p.d_twy, p.d_thr, p.W, p.H, p.nwx, p.nwy,             \
-                           p.plane_elems);                                                      \
+                           p.plane_elems, skip);                                                \
         break;                                                                                  \
       case 2:                                                                                   \
         hipLaunchKernelGGL((wff_cols_kernel<P, 2>), dim3(grid), dim3(128), smem, s, p.d_A,     \
                            p.d_D, p.d_Hy, p.d_twy, p.d_thr, p.W, p.H, p.nwx, p.nwy,             \
-                           p.plane_elems);                                                      \
+                           p.plane_elems, skip);                                                \
         break;                                                                                  \
       default:                                                                                  \
         hipLaunchKernelGGL((wff_cols_kernel<P, 4>), dim3(grid), dim3(256), smem, s, p.d_A,     \
                            p.d_D, p.d_Hy, p.d_twy, p.d_thr, p.W, p.H, p.nwx, p.nwy,             \
-                           p.plane_elems);                                                      \
+                           p.plane_elems, skip);                                                \
         break;                                                                                  \
     }                                                                                           \
   }

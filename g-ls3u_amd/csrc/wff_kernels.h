@@ -102,8 +102,10 @@ __device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ f,
 template <class P>
 __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_kernel(
     const float2* __restrict__ f, float2* __restrict__ A, const float2* __restrict__ Hx,
-    const float2* __restrict__ tw_g, int W, int H, int nwx, size_t plane) {
+    const float2* __restrict__ tw_g, int W, int H, int nwx, size_t plane,
+    const int* __restrict__ skip) {
   extern __shared__ float2 smem[];
+  if (skip && *skip) return;  // device-side decision (learned path): not needed
   const int y = xcd_remap(blockIdx.x, gridDim.x);  // uniform per workgroup
   if (y >= H) return;
   rows_fwd_body<P>(f, A, Hx, tw_g, W, H, nwx, plane, y, threadIdx.x % P::T, smem);
@@ -183,9 +185,10 @@ template <class P, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
     const float2* __restrict__ A, float2* __restrict__ D, const float2* __restrict__ Hy,
     const float2* __restrict__ tw_g, const float* __restrict__ thr_p, int W, int H, int nwx,
-    int nwy, size_t plane) {
+    int nwy, size_t plane, const int* __restrict__ skip) {
   constexpr int T = P::T, E = P::E;
   extern __shared__ float2 smem[];
+  if (skip && *skip) return;  // device-side decision (learned path): not needed
   float2* ttab = smem;
   FftLane<P, true>::fill_ttab(ttab, tw_g);
   __syncthreads();
@@ -280,8 +283,9 @@ template <class P, int MODE>
 __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_inv_kernel(
     const float2* __restrict__ D, const float2* __restrict__ Hx, const float2* __restrict__ tw_g,
     int W, int H, int nwx, size_t plane, float scale, float2* __restrict__ filtered,
-    UpdateArgs u) {
+    UpdateArgs u, const int* __restrict__ skip) {
   extern __shared__ float2 smem[];
+  if (skip && *skip) return;  // device-side decision (learned path): not needed
   const int y = xcd_remap(blockIdx.x, gridDim.x);
   if (y >= H) return;
   rows_inv_body<P, MODE>(D, Hx, tw_g, W, H, nwx, plane, scale, filtered, u, y,

@@ -3,14 +3,14 @@
 
 namespace glsu {
 
-hipError_t launch_rows_fwd(const WffPlan& p, const float2* f, hipStream_t s) {
+hipError_t launch_rows_fwd(const WffPlan& p, const float2* f, hipStream_t s, const int* skip) {
   const int rw = p.rows_waves;
   const int smem = plan_smem_bytes(p.planx, rw);
   switch (p.planx) {
 #define BODY                                                                              \
   if constexpr (P::T == 64) {                                                             \
     hipLaunchKernelGGL((wff_rows_fwd_kernel<P>), dim3(p.H), dim3(64 * rw), smem, s, f,    \
-                       p.d_A, p.d_Hx, p.d_twx, p.W, p.H, p.nwx, p.plane_elems);           \
+                       p.d_A, p.d_Hx, p.d_twx, p.W, p.H, p.nwx, p.plane_elems, skip);     \
   } else {                                                                                \
     return hipErrorInvalidValue;                                                          \
   }

@@ -147,6 +147,7 @@ def lib():
         "glsu_unet_destroy": (I, [V]),
         "glsu_dpra_frame_learned": (I, [V, V, V, I, V, D, I, I, PD, PI]),
         "glsu_dpra_copy_net": (I, [V, V]),
+        "glsu_dpra_learned_stats": (I, [V, PI, PI, PD, PD, I]),
         "glsu_fft_plans": (I, [PI, PI, PI, PI, I, PI]),
     }
     for name, (res, args) in sig.items():
@@ -182,7 +183,7 @@ def exported_symbols():
         "glsu_dpra_batch_destroy", "glsu_unet_create", "glsu_unet_param_count",
         "glsu_unet_get_params", "glsu_unet_set_params", "glsu_unet_get_grads", "glsu_unet_forward",
         "glsu_unet_loss_grad", "glsu_unet_adam_step", "glsu_unet_destroy",
-        "glsu_dpra_frame_learned", "glsu_dpra_copy_net", "glsu_fft_plans",
+        "glsu_dpra_frame_learned", "glsu_dpra_copy_net", "glsu_dpra_learned_stats", "glsu_fft_plans",
     ]
 
 

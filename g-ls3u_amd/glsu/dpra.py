@@ -114,11 +114,13 @@ class DPRA:
         check(lib().glsu_dpra_time_kernel(self._h, int(stage), int(reps), ctypes.byref(ms)))
         return ms.value
 
-    def frame_learned(self, frame, net, dphi=None, tol=0.03, tile=(0, 0)):
+    def frame_learned(self, frame, net, dphi=None, tol=0.03, tile=(0, 0), sync=True):
         """dpra_per_frame with the U-Net `net` (glsu.learned.UNet) in place of
         WFF2 and a WFF2 spot check on the 64x64 tile at `tile` (x, y): if the
         network's phase there differs from WFF2's by more than `tol` rad RMS,
-        the frame is filtered by WFF2 instead.  Returns (residual, fallback)."""
+        the frame is filtered by WFF2 instead.  Returns (residual, fallback);
+        with sync=False the call does not wait for the decision (it is taken
+        on the device) and returns None (see learned_stats)."""
         dt = self._check_frame(frame)
         if dphi is None:
             dphi = torch.empty((self.height, self.width), dtype=torch.float32, device=frame.device)
@@ -126,9 +128,18 @@ class DPRA:
         fb = ctypes.c_int()
         check(lib().glsu_dpra_frame_learned(
             self._h, net._h, ctypes.c_void_p(frame.data_ptr()), dt, ctypes.c_void_p(dphi.data_ptr()),
-            float(tol), int(tile[0]), int(tile[1]), ctypes.byref(res), ctypes.byref(fb)))
+            float(tol), int(tile[0]), int(tile[1]), ctypes.byref(res) if sync else None,
+            ctypes.byref(fb) if sync else None))
         self.last_dphi = dphi
-        return res.value, bool(fb.value)
+        return (res.value, bool(fb.value)) if sync else None
+
+    def learned_stats(self, reset=False):
+        """Spot-check statistics of the learned frames since the last reset."""
+        f, b = ctypes.c_int(), ctypes.c_int()
+        m, x = ctypes.c_double(), ctypes.c_double()
+        check(lib().glsu_dpra_learned_stats(self._h, ctypes.byref(f), ctypes.byref(b),
+                                            ctypes.byref(m), ctypes.byref(x), int(bool(reset))))
+        return dict(frames=f.value, fallbacks=b.value, mean_rms=m.value, max_rms=x.value)
 
     def last_net(self):
         z = torch.empty((self.height, self.width), dtype=torch.complex64, device="cuda")

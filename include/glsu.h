@@ -237,6 +237,12 @@ int glsu_unet_destroy(glsu_unet *h);
 int glsu_dpra_frame_learned(glsu_dpra *h, glsu_unet *net, const void *d_frame, int frame_dtype,
                             float *d_dphi, double tol, int tile_x, int tile_y, double *residual,
                             int *fallback);
+/* With residual and fallback both NULL the call is asynchronous: the spot
+ * check's decision stays on the device and selects which of the two enqueued
+ * paths runs.  Spot-check statistics since the last reset (no counterpart in
+ * the reference): checked frames, fallbacks, mean and max tile RMS. */
+int glsu_dpra_learned_stats(glsu_dpra *h, int *frames, int *fallbacks, double *mean_rms,
+                            double *max_rms, int reset);
 /* The network output (complex64, height*width) of the last learned frame. */
 int glsu_dpra_copy_net(glsu_dpra *h, void *d_out);
 

@@ -125,6 +125,18 @@ def test_learned_frame_fallback_and_net_paths():
         say(f"[learned] tol 0 frame {t}: residual {res:.3e} fallback {fb}")
         assert fb and res > 0
     assert np.array_equal(np.stack(outs), ref)
+    # asynchronous calls: the decision stays on the device; tol 0 -> every
+    # frame falls back, identical to the plain chain, and the statistics count it
+    d3 = glsu.DPRA(phi0)
+    outs3 = []
+    for t in range(2):
+        assert d3.frame_learned(frames[t], net, tol=0.0, tile=(32, 160), sync=False) is None
+        d3.update_ref_phi()
+        outs3.append(d3.last_dphi.cpu().numpy())
+    assert np.array_equal(np.stack(outs3), ref)
+    st = d3.learned_stats()
+    say(f"[learned] async stats {st}")
+    assert st["frames"] == 2 and st["fallbacks"] == 2 and st["max_rms"] > 0
     # spot checks at the corners: the window is clipped by the image border
     for tile in [(0, 0), (W - 64, H - 64), (0, H - 64)]:
         res, fb = glsu.DPRA(phi0).frame_learned(frames[0], net, tol=1e9, tile=tile)

@@ -253,7 +253,8 @@ __global__ __launch_bounds__(256) void conv3_dw_kernel(
 
 // ------------------------------------------------------------------------
 // Input layer: 2 channels (x = (cos, sin), float2) -> C0 channels, 3x3, relu.
-// One thread per pixel and 8 output channels; weights [C0][9][2] in LDS.
+// One thread per pixel computing all C0 outputs (the 9 neighbours are loaded
+// once); weights [C0][9][2] and biases in LDS, read as broadcasts.
 __global__ __launch_bounds__(256) void conv3_in_kernel(const float2* __restrict__ x,
                                                        const float* __restrict__ w,
                                                        const float* __restrict__ b,
@@ -262,12 +263,9 @@ __global__ __launch_bounds__(256) void conv3_in_kernel(const float2* __restrict_
   extern __shared__ float sw[];  // C0 * 18 weights then C0 biases
   for (int i = threadIdx.x; i < C0 * 19; i += blockDim.x) sw[i] = i < C0 * 18 ? w[i] : b[i - C0 * 18];
   __syncthreads();
-  const int groups = C0 / 8;
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long P = (long)n * H * W;
-  if (gid >= P * groups) return;
-  const long p = gid / groups;
-  const int g = (int)(gid % groups);
+  if (p >= P) return;
   const int xx = (int)(p % W), yy = (int)((p / W) % H), img = (int)(p / ((long)W * H));
   float in[18];
 #pragma unroll
@@ -278,16 +276,18 @@ __global__ __launch_bounds__(256) void conv3_in_kernel(const float2* __restrict_
     in[2 * t] = v.x;
     in[2 * t + 1] = v.y;
   }
-  bf16x8 o;
+  for (int g = 0; g < C0 / 8; ++g) {
+    bf16x8 o;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int co = g * 8 + k;
-    float s = sw[C0 * 18 + co];
+    for (int k = 0; k < 8; ++k) {
+      const int co = g * 8 + k;
+      float s = sw[C0 * 18 + co];
 #pragma unroll
-    for (int i = 0; i < 18; ++i) s = fmaf(sw[co * 18 + i], in[i], s);
-    o[k] = (__bf16)fmaxf(s, 0.f);
+      for (int i = 0; i < 18; ++i) s = fmaf(sw[co * 18 + i], in[i], s);
+      o[k] = (__bf16)fmaxf(s, 0.f);
+    }
+    *(bf16x8*)(Y + (size_t)p * C0 + g * 8) = o;
   }
-  *(bf16x8*)(Y + (size_t)p * C0 + g * 8) = o;
 }
 
 // weight/bias gradient of the input layer: dW[co][t][c] = sum_p dY[p][co] x[p+off_t][c]
@@ -578,7 +578,7 @@ hipError_t unet_conv3_dw(const __bf16* dY, int Cout, const __bf16* X1, int C1, c
 
 hipError_t unet_conv3_in(const float2* x, const float* w, const float* b, __bf16* Y, int n,
                          int H, int W, int C0, hipStream_t s) {
-  const long work = (long)n * H * W * (C0 / 8);
+  const long work = (long)n * H * W;
   hipLaunchKernelGGL(conv3_in_kernel, dim3(blocks(work)), dim3(256), C0 * 19 * sizeof(float), s,
                      x, w, b, Y, n, H, W, C0);
   return hipGetLastError();

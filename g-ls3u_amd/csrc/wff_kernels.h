@@ -51,7 +51,7 @@ __device__ __forceinline__ size_t tile_index(int y, int x, int H) {
 // Wave 0 computes Zr = FFT_x(f row) and shares it through LDS; wave w then
 // produces A_wx = IFFT_x(Zr * Hx_wx) for wx = w, w + RW, ...
 template <class P>
-__device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ f,
+__device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ frow,
                                               float2* __restrict__ A,
                                               const float2* __restrict__ Hx,
                                               const float2* __restrict__ tw_g, int W, int H,
@@ -70,7 +70,7 @@ __device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ f,
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       const int x = lane + T * j;
-      z[j] = x < W ? f[(size_t)y * W + x] : make_float2(0.f, 0.f);
+      z[j] = x < W ? frow[x] : make_float2(0.f, 0.f);
     }
     fft_fwd<P>(z, buf, ctx);
 #pragma unroll
@@ -108,7 +108,8 @@ __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_kernel(
   if (skip && *skip) return;  // device-side decision (learned path): not needed
   const int y = xcd_remap(blockIdx.x, gridDim.x);  // uniform per workgroup
   if (y >= H) return;
-  rows_fwd_body<P>(f, A, Hx, tw_g, W, H, nwx, plane, y, threadIdx.x % P::T, smem);
+  rows_fwd_body<P>(f + (size_t)y * W, A, Hx, tw_g, W, H, nwx, plane, y, threadIdx.x % P::T,
+                   smem);
 }
 
 // -------------------------------------------------------------- columns
@@ -332,8 +333,8 @@ __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_batch_kernel(
     const int r = xcd_remap(v, nrows);
     const BatchSeg& s = tab[seg_find<&BatchSeg::row0>(tab, nseg, r)];
     __syncthreads();  // LDS of the previous row is free
-    rows_fwd_body<P>(s.z, s.A, Hx, tw_g, s.W, s.H, nwx, s.plane, r - s.row0, opaque_lane<P>(),
-                     smem);
+    rows_fwd_body<P>(s.z + (size_t)(r - s.row0) * s.W, s.A, Hx, tw_g, s.W, s.H, nwx, s.plane,
+                     r - s.row0, opaque_lane<P>(), smem);
   }
 }
 

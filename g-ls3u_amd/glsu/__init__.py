@@ -11,5 +11,5 @@ from .params import wff_params, dpra_reference_params, wft2_default_params  # no
 from .wft2 import WFT2  # noqa: F401
 from .dpra import DPRA, gen_frames  # noqa: F401
 from .batch import DPRABatch  # noqa: F401
-from . import synth, autotune, learned  # noqa: F401
+from . import synth, autotune, learned, io  # noqa: F401
 from .learned import UNet  # noqa: F401

@@ -148,6 +148,10 @@ def lib():
         "glsu_dpra_frame_learned": (I, [V, V, V, I, V, D, I, I, PD, PI]),
         "glsu_dpra_copy_net": (I, [V, V]),
         "glsu_dpra_learned_stats": (I, [V, PI, PI, PD, PD, I]),
+        "glsu_read_matrix_bin": (I, [ctypes.c_char_p, I, PI, PI, V, ctypes.c_size_t]),
+        "glsu_write_matrix_bin": (I, [ctypes.c_char_p, I, I, I, V]),
+        "glsu_read_complex_text": (I, [ctypes.c_char_p, PI, PI, V, ctypes.c_size_t]),
+        "glsu_write_complex_text": (I, [ctypes.c_char_p, I, I, V, I]),
         "glsu_fft_plans": (I, [PI, PI, PI, PI, I, PI]),
     }
     for name, (res, args) in sig.items():
@@ -183,7 +187,9 @@ def exported_symbols():
         "glsu_dpra_batch_destroy", "glsu_unet_create", "glsu_unet_param_count",
         "glsu_unet_get_params", "glsu_unet_set_params", "glsu_unet_get_grads", "glsu_unet_forward",
         "glsu_unet_loss_grad", "glsu_unet_adam_step", "glsu_unet_destroy",
-        "glsu_dpra_frame_learned", "glsu_dpra_copy_net", "glsu_dpra_learned_stats", "glsu_fft_plans",
+        "glsu_dpra_frame_learned", "glsu_dpra_copy_net", "glsu_dpra_learned_stats",
+        "glsu_read_matrix_bin", "glsu_write_matrix_bin", "glsu_read_complex_text",
+        "glsu_write_complex_text", "glsu_fft_plans",
     ]
 
 

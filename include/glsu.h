@@ -15,6 +15,7 @@
 #ifndef GLSU_H
 #define GLSU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -245,6 +246,23 @@ int glsu_dpra_learned_stats(glsu_dpra *h, int *frames, int *fallbacks, double *m
                             double *max_rms, int reset);
 /* The network output (complex64, height*width) of the last learned frame. */
 int glsu_dpra_copy_net(glsu_dpra *h, void *d_out);
+
+/* ---- the reference's matrix files ------------------------------------------
+ * Binary matrix, replaces ReadMatrixFromDisk / WriteMatrixToDisk<T>
+ * (Utils/matrixIO.inl:33-62, 77-104; header Utils/matrixIO.h:18-35): int32
+ * rows, int32 cols, rows*cols elements of elem_size bytes, row-major.  With
+ * data == NULL the read only returns the size; capacity is in bytes. */
+int glsu_read_matrix_bin(const char *path, int elem_size, int *rows, int *cols, void *data,
+                         size_t capacity);
+int glsu_write_matrix_bin(const char *path, int elem_size, int rows, int cols, const void *data);
+/* Complex text, replaces fftwComplexMatRead2D / fftwComplexMatWrite2D
+ * (Utils/Utils.cpp:80-193): "rows,cols" then "re,im" per element.  data:
+ * interleaved doubles; capacity in doubles; precision <= 0 keeps the stream
+ * default of the reference (6 significant digits). */
+int glsu_read_complex_text(const char *path, int *rows, int *cols, double *data,
+                           size_t capacity);
+int glsu_write_complex_text(const char *path, int rows, int cols, const double *data,
+                            int precision);
 
 /* ---- utilities ---------------------------------------------------------- */
 /* Seeded synthetic fringe frames, written to device memory (float32,

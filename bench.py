@@ -40,6 +40,8 @@ WORKLOADS = {
     "batch64": (None, 30, (10.0, -0.2, 0.2, 0.1), None),
     # learned WFF2 surrogate: 1024^2, 500 frames, noise std drawn from U[5, 25]
     "learned1024": (1024, 500, (10.0, -0.2, 0.2, 0.1), None),
+    # AIA initial phase: 1024^2, 20 randomly phase-shifted uint8 frames, noise 10
+    "aia1024": (1024, 20, None, None),
 }
 BATCH_SIZES = [256, 512, 768, 1024, 1536, 2048]
 
@@ -135,6 +137,94 @@ def bench_learned(args, glsu, torch, np):
         "spot_check_rms_rad": {"mean": round(st["mean_rms"], 5), "max": round(st["max_rms"], 5)},
         "accuracy_vs_fp64_wff2": acc,
     }
+    print(json.dumps(line), flush=True)
+
+
+def aia_frames(np, M, H, W, seed, noise=10.0, amp=3.0, carrier=0.05):
+    """M uint8 frames I = 128 + 100 cos(phi + delta_k) + noise, phi = amp *
+    peaks + 2 pi carrier x, shifts uniform in (-pi, pi]."""
+    rng = np.random.default_rng(seed)
+    X, Y = np.meshgrid(np.linspace(-3, 3, W), np.linspace(-3, 3, H))
+    peaks = (3 * (1 - X) ** 2 * np.exp(-X ** 2 - (Y + 1) ** 2)
+             - 10 * (X / 5 - X ** 3 - Y ** 5) * np.exp(-X ** 2 - Y ** 2)
+             - np.exp(-(X + 1) ** 2 - Y ** 2) / 3)
+    phi = amp * peaks + 2 * np.pi * carrier * np.arange(W)[None, :]
+    delta = rng.uniform(-np.pi, np.pi, M)
+    f = np.empty((M, H, W), np.uint8)
+    for k in range(M):
+        v = 128 + 100 * np.cos(phi + delta[k]) + rng.normal(0, noise, (H, W))
+        f[k] = np.clip(np.rint(v), 0, 255)
+    return f
+
+
+def bench_aia(args, glsu, torch, np):
+    """AIA (the initial-phase estimator before the DPRA chain): one step = one
+    full solve with the reference's defaults (20 iterations at most, 1e-4,
+    AIA_CUDAF's error metric) on M uint8 frames resident in HBM.  value =
+    solves/s.  Roofline: the pass kernel (one launch per iteration reads every
+    frame once), timed with HIP events on its stream."""
+    size, M, _, _ = WORKLOADS["aia1024"]
+    W = H = args.size or size
+    M = args.frames or M
+    f = aia_frames(np, M, H, W, 4242)
+    frames = torch.from_numpy(f).cuda()
+    start = np.random.default_rng(7).normal(0, 1, M).astype(np.float32)
+    start[0] = 0
+    a = glsu.AIA(M, W, H)
+    phi = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    for _ in range(args.warmup):
+        a(frames, deltas=start, phi=phi)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, d, ms, it, err = a(frames, deltas=start, phi=phi)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    sps = args.steps / dt
+    it_ms = a.time_iter(frames, start, reps=20, stage=0)
+    pass_ms = a.time_iter(frames, start, reps=20, stage=1)
+    info = a.info()
+    nblk = info["blocks"] * info["chunks"]
+    # algorithmic bytes of one iteration launch: every frame once, plus the
+    # FP64 partials (5 + 2*KC per block) and the weights
+    b_alg = M * W * H + 8 * (5 + 2 * info["frames_per_chunk"]) * nblk + 8 * M
+    achieved = b_alg / (pass_ms * 1e-3) / 1e9
+    line = {
+        "metric": "AIA solves/s @1024^2 x 20 frames (uint8)", "value": round(sps, 2),
+        "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / max(args.steps, 1), 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"AIA initial phase, {W}x{H}, {M} uint8 frames with random "
+                               "shifts, noise 10; max_iter 20, max_err 1e-4, error metric "
+                               "max|delta - delta_old| (AIA_CUDAF)",
+                   "name": "aia1024", "iterations": it, "final_err": err,
+                   "launch": info},
+        "gpu_ms_per_solve": round(ms, 4),
+        "iteration_ms": round(it_ms, 5),
+        "roofline": {"bound": "hbm", "kernel": "aia_pass_kernel", "avg_ms": round(pass_ms, 5),
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "B_alg_per_launch": b_alg,
+                     "note": "frames are re-read every iteration and fit the 256 MB "
+                             "last-level cache at this size, so the achieved rate is an "
+                             "effective one"},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        nthreads = min(os.cpu_count() or 1, 16)
+        t1 = time.perf_counter()
+        p_ref, d_ref, it_ref, _ = O.aia(f.astype(np.float64), start.astype(np.float64), 20,
+                                        1e-4, 0, nthreads=nthreads)
+        cdt = time.perf_counter() - t1
+        e = np.angle(np.exp(1j * (phi.cpu().numpy().astype(np.float64) - p_ref)))
+        line["cpu_baseline"] = {
+            "value": round(1.0 / cdt, 4), "unit": "solves/s", "cores": nthreads, "kind": "port",
+            "sample": f"one full solve of the same frames, FP64 oracle (reference algorithm, "
+                      f"OpenMP over pixels), {it_ref} iterations, {cdt:.2f} s"}
+        line["parity"] = {"iterations_gpu": it, "iterations_oracle": it_ref,
+                          "phase_rms_rad": float(np.sqrt(np.mean(e * e))),
+                          "phase_max_abs_rad": float(np.abs(e).max()),
+                          "shift_max_abs_rad": float(np.abs(np.angle(np.exp(1j * (d - d_ref)))).max())}
     print(json.dumps(line), flush=True)
 
 
@@ -294,6 +384,8 @@ def main():
         return bench_batch(args, glsu, torch, np)
     if args.workload == "learned1024":
         return bench_learned(args, glsu, torch, np)
+    if args.workload == "aia1024":
+        return bench_aia(args, glsu, torch, np)
     size, T, wff, gen = WORKLOADS[args.workload]
     W = H = args.size or size
     T = args.frames or T

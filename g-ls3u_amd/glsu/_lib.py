@@ -18,6 +18,7 @@ ERR_PADDING = -2
 ERR_SIZE = -3
 ERR_DEVICE = -4
 ERR_UNSUPPORTED = -5
+ERR_NOT_PD = -6
 
 WFF = 0
 WFR = 1
@@ -34,6 +35,11 @@ class GlsuError(RuntimeError):
 
 class PaddingError(GlsuError):
     """Padded FFT size out of range; the reference constructors throw int -1."""
+
+
+class NotPositiveDefiniteError(GlsuError):
+    """An AIA normal matrix is not positive definite; the reference prints
+    that and exits (AIA/aia_cpuf.cpp:163-167)."""
 
 
 class Tuning(ctypes.Structure):
@@ -153,6 +159,12 @@ def lib():
         "glsu_read_complex_text": (I, [ctypes.c_char_p, PI, PI, V, ctypes.c_size_t]),
         "glsu_write_complex_text": (I, [ctypes.c_char_p, I, I, V, I]),
         "glsu_fft_plans": (I, [PI, PI, PI, PI, I, PI]),
+        "glsu_aia_create": (I, [PV, I, I, I, V]),
+        "glsu_aia_run": (I, [V, V, I, V, I, U64, I, D, I, V, PI, PD, PD]),
+        "glsu_aia_run_host": (I, [V, PV, I, V, I, U64, I, D, I, V, PI, PD, PD]),
+        "glsu_aia_info": (I, [V, PI, PI, PI]),
+        "glsu_aia_time_iter": (I, [V, V, I, V, I, I, PD]),
+        "glsu_aia_destroy": (I, [V]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -167,6 +179,8 @@ def check(rc):
         msg = lib().glsu_last_error().decode(errors="replace")
         if rc == ERR_PADDING:
             raise PaddingError(rc, msg)
+        if rc == ERR_NOT_PD:
+            raise NotPositiveDefiniteError(rc, msg)
         raise GlsuError(rc, msg)
     return rc
 
@@ -190,6 +204,7 @@ def exported_symbols():
         "glsu_dpra_frame_learned", "glsu_dpra_copy_net", "glsu_dpra_learned_stats",
         "glsu_read_matrix_bin", "glsu_write_matrix_bin", "glsu_read_complex_text",
         "glsu_write_complex_text", "glsu_fft_plans",
+        "glsu_aia_create", "glsu_aia_run", "glsu_aia_run_host", "glsu_aia_info", "glsu_aia_time_iter", "glsu_aia_destroy",
     ]
 
 

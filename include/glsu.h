@@ -28,6 +28,8 @@ extern "C" {
 #define GLSU_ERR_SIZE (-3)        /* frame size differs from the initialised size    */
 #define GLSU_ERR_DEVICE (-4)      /* HIP runtime error                               */
 #define GLSU_ERR_UNSUPPORTED (-5) /* mode not provided by this library (WFR2)        */
+#define GLSU_ERR_NOT_PD (-6)      /* AIA normal matrix not positive definite (ref:
+                                     prints and exits, AIA/aia_cpuf.cpp:163-167)    */
 
 /* WFT_FPA::WFT::WFT_TYPE, Utils/WFT-FPA.h:72-76 */
 #define GLSU_WFF 0
@@ -246,6 +248,44 @@ int glsu_dpra_learned_stats(glsu_dpra *h, int *frames, int *fallbacks, double *m
                             double *max_rms, int reset);
 /* The network output (complex64, height*width) of the last learned frame. */
 int glsu_dpra_copy_net(glsu_dpra *h, void *d_out);
+
+/* ---- AIA: initial phase from randomly phase-shifted frames -----------------
+ * Replaces AIA::AIA_CUDAF (AIA/aia_cudaf.h:15-72): the constructor
+ * AIA_CUDAF(iM, icols, irows) (:23-25, AIA/aia_cudaf.cu:248-285) and
+ * operator() (:28-38, AIA/aia_cudaf.cu:290-372); the same algorithm as
+ * AIA_CPU_Dn / AIA_CPU_DnF (AIA/aia_cpu.h:10, AIA/aia_cpuf.h:15) and
+ * MatLab/fp_aia.m.  nframes: 3..1024. */
+typedef struct glsu_aia glsu_aia;
+int glsu_aia_create(glsu_aia **out, int nframes, int width, int height, void *hip_stream);
+/* d_frames: nframes*height*width elements of frame_dtype, frame-major (device).
+ * deltas (host, nframes): in, the initial shifts when use_given != 0, else
+ * delta_0 = 0 and N(0,1) draws from `seed` (the reference draws them from a
+ * clock seed when v_deltas.size() != M, AIA/aia_cpuf.cpp:31-45); out, the
+ * final shifts minus delta_0, wrapped (AIA/aia_cudaf.cu:364-371).
+ * Iterates while err > max_err and iters < max_iter (err starts at
+ * 2*max_err).  err_mode 0: err = max_k |delta_k - delta_k_old| (AIA_CUDAF,
+ * aia_cudaf.cu:448-461; AIA_CPU_Dn; fp_aia.m:61); 1: the same on shifts
+ * relative to delta_0 (AIA_CPU_DnF, aia_cpuf.cpp:247-260; AIA_CUDAF_YC2).
+ * d_phi (device, height*width floats): phi + delta_0 wrapped into (-pi, pi]
+ * (aia_cudaf.cu:351-356).  *ms: GPU time of the whole solve.  Returns
+ * GLSU_ERR_NOT_PD when a normal matrix is not positive definite. */
+int glsu_aia_run(glsu_aia *h, const void *d_frames, int frame_dtype, float *deltas, int use_given,
+                 uint64_t seed, int max_iter, double max_err, int err_mode, float *d_phi,
+                 int *iters, double *err, double *ms);
+/* The same from host frames (one pointer per frame, as the reference copies
+ * each cv::Mat, aia_cudaf.cu:320-324) to a host phase map. */
+int glsu_aia_run_host(glsu_aia *h, const void *const *h_frames, int frame_dtype, float *deltas,
+                      int use_given, uint64_t seed, int max_iter, double max_err, int err_mode,
+                      float *h_phi, int *iters, double *err, double *ms);
+/* Average GPU time over reps repetitions, with the convergence test off, from
+ * the given initial shifts, of one iteration (stage 0: the pass launch that
+ * does the phase step and the shift-step sums, plus the one-block solve) or of
+ * the pass launch alone (stage 1).  No reference counterpart (measurement). */
+int glsu_aia_time_iter(glsu_aia *h, const void *d_frames, int frame_dtype, const float *deltas,
+                       int stage, int reps, double *ms);
+/* Launch shape: blocks per frame chunk, chunks, frames per chunk. */
+int glsu_aia_info(glsu_aia *h, int *blocks, int *chunks, int *frames_per_chunk);
+int glsu_aia_destroy(glsu_aia *h);
 
 /* ---- the reference's matrix files ------------------------------------------
  * Binary matrix, replaces ReadMatrixFromDisk / WriteMatrixToDisk<T>

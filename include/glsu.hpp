@@ -3,6 +3,7 @@
 //   WFT_FPA::WFT::WFT2_CUDAF  (WFT/WFT2_CUDAf.h:11-40)   -> WFT_FPA::WFT::WFT2_HIPF
 //   WFT_FPA::WFT::WFT2_DeviceResultsF (WFT/WFT.h:114-135) -> same name (WFF member)
 //   DPRA::DPRA_CUDAF          (DPRA/dpra_cudaf.h:21-45)   -> DPRA::DPRA_HIPF
+//   AIA::AIA_CUDAF            (AIA/aia_cudaf.h:15-72)     -> AIA::AIA_HIPF
 // Error behaviour follows the reference: constructors throw int -1 when the
 // padded FFT size is out of range (WFT/WFT2_CPU.cpp:69-73); other failures,
 // which the reference prints or ignores, throw std::runtime_error.
@@ -153,3 +154,52 @@ class DPRA_HIPF {
 };
 
 }  // namespace DPRA
+
+namespace AIA {
+
+// AIA::AIA_CUDAF (AIA/aia_cudaf.h:15-72) -> AIA::AIA_HIPF: the initial phase
+// and the phase shifts of iM >= 3 randomly phase-shifted frames.
+class AIA_HIPF {
+ public:
+  AIA_HIPF() = delete;
+  AIA_HIPF(const AIA_HIPF&) = delete;
+  AIA_HIPF& operator=(const AIA_HIPF&) = delete;
+
+  // AIA_CUDAF(iM, icols, irows), AIA/aia_cudaf.h:23-25.
+  AIA_HIPF(int iM, int icols, int irows, hipStream_t stream = nullptr)
+      : m_(iM), w_(icols), h_(irows) {
+    glsu_detail::check(glsu_aia_create(&h, iM, icols, irows, stream));
+  }
+  ~AIA_HIPF() { glsu_aia_destroy(h); }
+
+  // operator()(v_phi, v_deltas, runningtime, iters, err, v_f, iMaxIterations,
+  // dMaxErr, iNumThreads), AIA/aia_cudaf.h:28-38: host frames of icols*irows
+  // bytes (the reference's cv::Mat data); v_deltas holds the initial shifts when
+  // its size is iM, else they are drawn (delta_0 = 0, N(0,1) from `seed`; the
+  // reference seeds from the clock).  err_mode 0 is AIA_CUDAF's error metric,
+  // 1 that of AIA_CPU_DnF.  A normal matrix that is not positive definite
+  // throws std::runtime_error (the reference prints and exits).
+  void operator()(std::vector<float>& v_phi, std::vector<float>& v_deltas, double& runningtime,
+                  int& iters, float& err, const std::vector<const std::uint8_t*>& v_f,
+                  int iMaxIterations = 20, float dMaxErr = 1e-4f, int /*iNumThreads*/ = 1,
+                  int err_mode = 0, unsigned long long seed = 0) {
+    if ((int)v_f.size() != m_) throw std::runtime_error("glsu: AIA frame count differs");
+    const int given = (int)v_deltas.size() == m_;
+    v_deltas.resize(m_);
+    v_phi.resize((size_t)w_ * h_);
+    std::vector<const void*> f(v_f.begin(), v_f.end());
+    double e = 0;
+    glsu_detail::check(glsu_aia_run_host(h, f.data(), GLSU_U8, v_deltas.data(), given, seed,
+                                         iMaxIterations, dMaxErr, err_mode, v_phi.data(), &iters,
+                                         &e, &runningtime));
+    err = (float)e;
+  }
+
+  glsu_aia* handle() const { return h; }
+
+ private:
+  glsu_aia* h = nullptr;
+  int m_, w_, h_;
+};
+
+}  // namespace AIA

@@ -7,6 +7,7 @@
  *   phase / ref update      DPRA/dpra_cpu.cpp:242-258
  *   frame loop              DPRA/dpra_cpu.cpp:58-89
  *   OPT padding table       WFT/WFT.h:18-32, WFT/WFT.cpp:10-23, MatLab/FFT_OptimalSize.m:1-23
+ *   AIA (initial phase)     AIA/aia_cpuf.cpp:16-260, AIA/aia_cudaf.cu:290-461, MatLab/fp_aia.m
  *
  * The reference binds two third-party routines on this path (Intel MKL, version
  * not pinned in the repository: DPRA/DPRA.vcxproj <UseIntelMKL>Parallel):
@@ -31,6 +32,10 @@
  *   DPRA LS        noise-free closed form; numpy.linalg.solve brute force incl.
  *                  border pixels (A00 = 9 quirk); offset/gain invariance.
  *   update         closed-form cumulative sums for a WFF-free noise-free chain.
+ *   AIA            the reference's Test/Phi_CPU.csv (per-pixel phase-step solutions on
+ *                  Test/{1,2,3}.bmp); noise-free closed forms of each half-step and of
+ *                  the loop; three-frame exact fit; shift-offset gauge; loop rule and
+ *                  error metrics against half-steps driven from Python.
  * Parity unpinned: MKL's exact rounding (FFTW_ESTIMATE plans, dsysv pivots) and
  * what dsysv leaves in b on singular pixels; the oracle defines that case as
  * Delta = 0 (|det| <= 1e-10 * A00*A11*A22).
@@ -746,4 +751,184 @@ void orc_wrapped_err(const double *a, const double *b, int64_t n, double *rms, d
     }
     *rms = n ? sqrt(s / (double)n) : 0;
     *maxabs = mx;
+}
+
+/* ========================================================================== */
+/* AIA: advanced iterative algorithm for phase extraction from randomly        */
+/* phase-shifted frames (Z. Wang, B. Han, Opt. Lett. 29, 1671 (2004)).        */
+/* Restates MatLab/fp_aia.m:33-121, AIA/aia_cpu.cpp, AIA/aia_cpuf.cpp:16-260  */
+/* and AIA/aia_cudaf.cu:290-461 in FP64.  Model I_k(j) = a_j + b_j cos(phi_j  */
+/* + delta_k); each half-step is a 3x3 least squares in (a, b cos, -b sin).  */
+/* ========================================================================== */
+
+/* Cholesky A = U^T U of a symmetric 3x3 (upper triangle of the column-major
+ * array read), as LAPACKE_?posv(.., 'U', ..) factors it (aia_cpuf.cpp:161,238).
+ * Returns 0, or k (1..3) when the leading minor of order k is not positive
+ * definite -- the reference prints that and exits (aia_cpuf.cpp:163-167).  A
+ * pivot at or below 1e-10 of its diagonal entry counts as not positive (LAPACK
+ * tests <= 0, but rounding can leave a singular matrix, e.g. all shifts
+ * equal, a tiny positive pivot). */
+#define ORC_CHOL_TOL 1e-10
+static int chol3(const double *A, double U[3][3]) {
+    const double a00 = A[0], a01 = A[3], a02 = A[6], a11 = A[4], a12 = A[7], a22 = A[8];
+    memset(U, 0, 9 * sizeof(double));
+    if (!(a00 > 0)) return 1;
+    U[0][0] = sqrt(a00);
+    U[0][1] = a01 / U[0][0];
+    U[0][2] = a02 / U[0][0];
+    double d1 = a11 - U[0][1] * U[0][1];
+    if (!(d1 > ORC_CHOL_TOL * a11)) return 2;
+    U[1][1] = sqrt(d1);
+    U[1][2] = (a12 - U[0][1] * U[0][2]) / U[1][1];
+    double d2 = a22 - U[0][2] * U[0][2] - U[1][2] * U[1][2];
+    if (!(d2 > ORC_CHOL_TOL * a22)) return 3;
+    U[2][2] = sqrt(d2);
+    return 0;
+}
+
+static void chol3_solve(double U[3][3], double *b) {
+    /* U^T y = b */
+    double y0 = b[0] / U[0][0];
+    double y1 = (b[1] - U[0][1] * y0) / U[1][1];
+    double y2 = (b[2] - U[0][2] * y0 - U[1][2] * y1) / U[2][2];
+    /* U x = y */
+    double x2 = y2 / U[2][2];
+    double x1 = (y1 - U[1][2] * x2) / U[1][1];
+    double x0 = (y0 - U[0][1] * x1 - U[0][2] * x2) / U[0][0];
+    b[0] = x0; b[1] = x1; b[2] = x2;
+}
+
+int orc_aia_chol3_solve(const double *A, double *b) {
+    double U[3][3];
+    int info = chol3(A, U);
+    if (info) return info;
+    chol3_solve(U, b);
+    return 0;
+}
+
+/* computePhi, AIA/aia_cpuf.cpp:112-183 (MatLab/fp_aia.m:74-101):
+ * A1 = sum_k v_k v_k^T with v_k = (1, cos delta_k, sin delta_k); per pixel
+ * b = sum_k I_k v_k, x = A1^-1 b, phi = atan2(-x2, x1).  f: M x N frames
+ * (frame-major).  X (optional): N x 3 solutions, as the reference's
+ * v_b_phi after the solve (aia_cpuf.cpp:161, dumped to Test/Phi_CPU.csv by the
+ * commented lines :169-178).  Returns 0 or the LAPACK-style info. */
+int orc_aia_phi(const double *f, int M, int64_t N, const double *delta, double *phi,
+                double *X, int nthreads) {
+    double A[9] = {0};
+    double *c = malloc(sizeof(double) * (size_t)M), *s = malloc(sizeof(double) * (size_t)M);
+    for (int k = 0; k < M; k++) {
+        c[k] = cos(delta[k]);
+        s[k] = sin(delta[k]);
+        A[3] += c[k]; A[4] += c[k] * c[k];
+        A[6] += s[k]; A[7] += c[k] * s[k]; A[8] += s[k] * s[k];
+    }
+    A[0] = M;
+    double U[3][3];
+    int info = chol3(A, U);
+    if (!info) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (int64_t j = 0; j < N; j++) {
+            double b[3] = {0, 0, 0};
+            for (int k = 0; k < M; k++) {
+                double I = f[(size_t)k * N + j];
+                b[0] += I;
+                b[1] += I * c[k];
+                b[2] += I * s[k];
+            }
+            chol3_solve(U, b);
+            if (X) { X[3 * j] = b[0]; X[3 * j + 1] = b[1]; X[3 * j + 2] = b[2]; }
+            phi[j] = atan2(-b[2], b[1]);
+        }
+    }
+    free(c);
+    free(s);
+    return info;
+}
+
+/* Pixel sums in a fixed order whatever the thread count: 256 static chunks,
+ * each summed in sequence, then the chunks in order. */
+#define ORC_AIA_CHUNKS 256
+
+/* computeDelta, AIA/aia_cpuf.cpp:185-245 (MatLab/fp_aia.m:103-121):
+ * A2 = sum_j u_j u_j^T with u_j = (1, cos phi_j, sin phi_j); per frame
+ * b = sum_j I_k(j) u_j, x = A2^-1 b, delta_k = atan2(-x2, x1). */
+int orc_aia_delta(const double *f, int M, int64_t N, const double *phi, double *delta,
+                  int nthreads) {
+    const int V = 5 + 3 * M;  /* sc, scc, ss, scs, sss, then (t, p, q) per frame */
+    double *part = calloc((size_t)ORC_AIA_CHUNKS * V, sizeof(double));
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int ch = 0; ch < ORC_AIA_CHUNKS; ch++) {
+        int64_t j0 = N * ch / ORC_AIA_CHUNKS, j1 = N * (ch + 1) / ORC_AIA_CHUNKS;
+        double *acc = part + (size_t)ch * V;
+        for (int64_t j = j0; j < j1; j++) {
+            double cp = cos(phi[j]), sp = sin(phi[j]);
+            acc[0] += cp; acc[1] += cp * cp; acc[2] += sp; acc[3] += cp * sp; acc[4] += sp * sp;
+            for (int k = 0; k < M; k++) {
+                double I = f[(size_t)k * N + j];
+                acc[5 + 3 * k] += I;
+                acc[6 + 3 * k] += I * cp;
+                acc[7 + 3 * k] += I * sp;
+            }
+        }
+    }
+    double *tot = calloc((size_t)V, sizeof(double));
+    for (int ch = 0; ch < ORC_AIA_CHUNKS; ch++)
+        for (int v = 0; v < V; v++) tot[v] += part[(size_t)ch * V + v];
+    double A[9] = {(double)N, 0, 0, tot[0], tot[1], 0, tot[2], tot[3], tot[4]};
+    double U[3][3];
+    int info = chol3(A, U);
+    if (!info)
+        for (int k = 0; k < M; k++) {
+            double b[3] = {tot[5 + 3 * k], tot[6 + 3 * k], tot[7 + 3 * k]};
+            chol3_solve(U, b);
+            delta[k] = atan2(-b[2], b[1]);
+        }
+    free(part);
+    free(tot);
+    return info;
+}
+
+/* operator(), AIA/aia_cpuf.cpp:16-107 and AIA/aia_cudaf.cu:290-372: iterate
+ * phi/delta estimation while err > max_err and iters < max_iter (err starts
+ * at 2*max_err), one more phi estimate, then phi += delta_0 and
+ * delta_k -= delta_0, both wrapped by atan2(sin, cos).  err_mode 0:
+ * max_k |delta_k - delta_k_old| (aia_cudaf.cu:448-461, aia_cpu.cpp:249-263,
+ * fp_aia.m:61); 1: max_k |(delta_k - delta_0) - (delta_k_old - delta_0_old)|
+ * (aia_cpuf.cpp:247-260, aia_cudaf_v2.cu:665-676).  delta: in/out (M).
+ * Returns 0 or the info of the failing factorisation. */
+int orc_aia(const double *f, int M, int64_t N, double *delta, int max_iter, double max_err,
+            int err_mode, double *phi, int *iters, double *err, int nthreads) {
+    double *old = malloc(sizeof(double) * (size_t)M);
+    int info = 0;
+    *err = max_err * 2.0;
+    *iters = 0;
+    while (*err > max_err && *iters < max_iter) {
+        memcpy(old, delta, sizeof(double) * (size_t)M);
+        if ((info = orc_aia_phi(f, M, N, delta, phi, NULL, nthreads))) goto out;
+        if ((info = orc_aia_delta(f, M, N, phi, delta, nthreads))) goto out;
+        ++*iters;
+        double e = 0;
+        for (int k = 0; k < M; k++) {
+            double d = err_mode ? fabs((delta[k] - delta[0]) - (old[k] - old[0]))
+                                : fabs(delta[k] - old[k]);
+            if (d > e) e = d;
+        }
+        *err = e;
+    }
+    if ((info = orc_aia_phi(f, M, N, delta, phi, NULL, nthreads))) goto out;
+    {
+        const double d0 = delta[0];
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (int64_t j = 0; j < N; j++) {
+            double t = phi[j] + d0;
+            phi[j] = atan2(sin(t), cos(t));
+        }
+        for (int k = M - 1; k >= 0; k--) {
+            double t = delta[k] - d0;
+            delta[k] = atan2(sin(t), cos(t));
+        }
+    }
+out:
+    free(old);
+    return info;
 }

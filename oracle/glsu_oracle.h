@@ -80,6 +80,17 @@ int  orc_dpra_run(orc_dpra *d, const double *frames, int T, double *dphi_all);
 int  orc_solve3_sym(const double *A, double *b);
 double orc_dpra_thr(const orc_dpra *d);
 
+/* ---- AIA (AIA/aia_cpuf.cpp, AIA/aia_cudaf.cu, MatLab/fp_aia.m) ---------------
+ * f: M frames of N pixels, frame-major.  Return 0, or k (1..3) when the leading
+ * minor of order k of a normal matrix is not positive definite. */
+int orc_aia_chol3_solve(const double *A, double *b);
+int orc_aia_phi(const double *f, int M, int64_t N, const double *delta, double *phi,
+                double *X, int nthreads);
+int orc_aia_delta(const double *f, int M, int64_t N, const double *phi, double *delta,
+                  int nthreads);
+int orc_aia(const double *f, int M, int64_t N, double *delta, int max_iter, double max_err,
+            int err_mode, double *phi, int *iters, double *err, int nthreads);
+
 /* ---- metrics -------------------------------------------------------------- */
 void orc_wrapped_err(const double *a, const double *b, int64_t n,
                      double *rms, double *maxabs);

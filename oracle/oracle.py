@@ -60,6 +60,10 @@ def lib():
             "orc_solve3_sym": (I, [_dp, _dp]),
             "orc_dpra_thr": (D, [V]),
             "orc_wrapped_err": (None, [_dp, _dp, ctypes.c_int64, _dp, _dp]),
+            "orc_aia_chol3_solve": (I, [_dp, _dp]),
+            "orc_aia_phi": (I, [_dp, I, ctypes.c_int64, _dp, _dp, _dp, I]),
+            "orc_aia_delta": (I, [_dp, I, ctypes.c_int64, _dp, _dp, I]),
+            "orc_aia": (I, [_dp, I, ctypes.c_int64, _dp, I, D, I, _dp, _ip, _dp, I]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -265,3 +269,52 @@ def wrapped_err(a, b):
     r, m = ctypes.c_double(), ctypes.c_double()
     lib().orc_wrapped_err(_ptr(a), _ptr(b), a.size, ctypes.byref(r), ctypes.byref(m))
     return r.value, m.value
+
+
+# ---------------------------------------------------------------- AIA
+def _frames2d(f):
+    f = _f64(f)
+    return np.ascontiguousarray(f.reshape(f.shape[0], -1)), f.shape[1:]
+
+
+def aia_phi(f, delta, nthreads=0, return_x=False):
+    """computePhi (AIA/aia_cpuf.cpp:112-183): phase of every pixel for the
+    given shifts; with return_x also the per-pixel solutions (N x 3)."""
+    F, shape = _frames2d(f)
+    d = _f64(delta)
+    phi = np.empty(F.shape[1])
+    X = np.empty((F.shape[1], 3)) if return_x else None
+    info = lib().orc_aia_phi(_ptr(F), F.shape[0], F.shape[1], _ptr(d), _ptr(phi),
+                             _ptr(X) if return_x else None, nthreads or os.cpu_count())
+    if info:
+        raise ArithmeticError(f"leading minor of order {info} is not positive definite")
+    return (phi.reshape(shape), X) if return_x else phi.reshape(shape)
+
+
+def aia_delta(f, phi, nthreads=0):
+    """computeDelta (AIA/aia_cpuf.cpp:185-245): shifts of every frame for the
+    given phase."""
+    F, _ = _frames2d(f)
+    p = _f64(phi).ravel()
+    d = np.empty(F.shape[0])
+    info = lib().orc_aia_delta(_ptr(F), F.shape[0], F.shape[1], _ptr(p), _ptr(d),
+                               nthreads or os.cpu_count())
+    if info:
+        raise ArithmeticError(f"leading minor of order {info} is not positive definite")
+    return d
+
+
+def aia(f, delta, max_iter=20, max_err=1e-4, err_mode=0, nthreads=0):
+    """AIA operator() (AIA/aia_cpuf.cpp:16-107, AIA/aia_cudaf.cu:290-372):
+    returns (phi, deltas, iters, err) with phi + delta_0 and delta - delta_0
+    wrapped."""
+    F, shape = _frames2d(f)
+    d = _f64(delta).copy()
+    phi = np.empty(F.shape[1])
+    it, err = ctypes.c_int(), ctypes.c_double()
+    info = lib().orc_aia(_ptr(F), F.shape[0], F.shape[1], _ptr(d), int(max_iter),
+                         float(max_err), int(err_mode), _ptr(phi), ctypes.byref(it),
+                         ctypes.byref(err), nthreads or os.cpu_count())
+    if info:
+        raise ArithmeticError(f"leading minor of order {info} is not positive definite")
+    return phi.reshape(shape), d, it.value, err.value

@@ -42,6 +42,9 @@ WORKLOADS = {
     "learned1024": (1024, 500, (10.0, -0.2, 0.2, 0.1), None),
     # AIA initial phase: 1024^2, 20 randomly phase-shifted uint8 frames, noise 10
     "aia1024": (1024, 20, None, None),
+    # WFR2 (ridge mode) with the reference's WFR defaults: sigma 10, [-2, 2] step
+    # 0.025 on both axes (161 x 161 pairs), one 1024^2 field per step
+    "wfr1024": (1024, 1, (10.0, -2.0, 2.0, 0.025), None),
 }
 BATCH_SIZES = [256, 512, 768, 1024, 1536, 2048]
 
@@ -228,6 +231,94 @@ def bench_aia(args, glsu, torch, np):
     print(json.dumps(line), flush=True)
 
 
+def wfr_field(np, W, H, seed, noise=0.3, amp=3.0, carrier=0.05):
+    """A fringe field b exp(j phi) + complex noise, phi = amp peaks + 2 pi carrier x."""
+    rng = np.random.default_rng(seed)
+    X, Y = np.meshgrid(np.linspace(-3, 3, W), np.linspace(-3, 3, H))
+    peaks = (3 * (1 - X) ** 2 * np.exp(-X ** 2 - (Y + 1) ** 2)
+             - 10 * (X / 5 - X ** 3 - Y ** 5) * np.exp(-X ** 2 - Y ** 2)
+             - np.exp(-(X + 1) ** 2 - Y ** 2) / 3)
+    ph = amp * peaks + 2 * np.pi * carrier * np.arange(W)[None, :]
+    f = (1 + 0.3 * np.cos(X)) * np.exp(1j * ph)
+    f = f + noise * (rng.normal(size=(H, W)) + 1j * rng.normal(size=(H, W)))
+    return f.astype(np.complex64)
+
+
+def bench_wfr(args, glsu, torch, np):
+    """WFR2 (the ridge mode of WFT2) with the reference's WFR defaults.  One
+    step = the eight ridge maps of one field.  value = fields/s.  Roofline:
+    the column kernel (ridge over wy), timed with HIP events on its stream."""
+    size, _, wfr, _ = WORKLOADS["wfr1024"]
+    W = H = args.size or size
+    f = torch.from_numpy(wfr_field(np, W, H, 77)).cuda()
+    params = glsu.wff_params(*wfr)
+    w = glsu.WFT2(W, H, type=glsu.WFR, params=params)
+    out = torch.empty((8, H, W), dtype=torch.float32, device="cuda")
+    for _ in range(args.warmup):
+        w.wfr(f, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        w.wfr(f, out=out)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    fps = args.steps / dt
+    info = w.info()
+    nwx, nwy, Ny = info["nwx"], info["nwy"], info["ny"]
+    stage_ms = [w.time_kernel(f, k, reps=3) for k in (1, 2, 3)]
+    col_ms = stage_ms[1]
+    # per (wx, column): 1 + nwy FFTs of length Ny, and per pair the spectrum
+    # product (6 flop/point) and the magnitude compare (4)
+    flops = nwx * W * ((1 + nwy) * 5.0 * Ny * math.log2(Ny) + nwy * Ny * 10.0)
+    achieved = flops / (col_ms * 1e-3) / 1e12
+    line = {
+        "metric": "WFR2 fields/s @1024^2 (161 x 161 pairs)", "value": round(fps, 3),
+        "unit": "fields/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / max(args.steps, 1), 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"WFR2 ridge maps of one {W}x{H} fringe field, sigma {wfr[0]:g}, "
+                               f"wx, wy in [{wfr[1]:g}, {wfr[2]:g}] step {wfr[3]:g} "
+                               f"({nwx} x {nwy} pairs; the reference's WFR defaults)",
+                   "name": "wfr1024", "fft_len": [info["nx"], Ny]},
+        "kernel_ms": {"rows_fwd": round(stage_ms[0], 4), "wfr_cols": round(stage_ms[1], 4),
+                      "ridge_curvature": round(stage_ms[2], 4)},
+        "roofline": {"bound": "mfma", "kernel": "wfr_cols_kernel", "avg_ms": round(col_ms, 4),
+                     "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "note": "FFT butterflies on the FP32 vector ALU, priced against the FP32 peak"},
+    }
+    if not args.no_cpu:
+        # bounded sample: the FP64 oracle over the first k wy rows of the grid
+        # (k * nwx of the nwx * nwy pairs; the oracle runs one wy row per
+        # thread, so k = threads), scaled to whole fields per second; parity on
+        # the same sample against the GPU run with the same grid
+        from oracle import oracle as O
+        nthreads = min(os.cpu_count() or 1, 16)
+        k = nthreads
+        wyh = wfr[1] + (k - 1) * wfr[3]
+        fh = f.cpu().numpy()
+        o = O.WFR2(W, H, sigx=wfr[0], wxl=wfr[1], wxh=wfr[2], wxi=wfr[3],
+                   wyl=wfr[1], wyh=wyh, wyi=wfr[3], nthreads=nthreads)
+        t1 = time.perf_counter()
+        ref = o(fh.astype(np.complex128))
+        cdt = time.perf_counter() - t1
+        ws = glsu.WFT2(W, H, type=glsu.WFR,
+                       params=glsu.wff_params(wfr[0], wfr[1], wfr[2], wfr[3], wyl=wfr[1],
+                                              wyh=wyh, wyi=wfr[3]))
+        zs = {a: b.cpu().numpy().astype(np.float64) for a, b in ws.wfr(f).items()}
+        same = (np.abs(zs["wx"] - ref["wx"]) < 1e-6) & (np.abs(zs["wy"] - ref["wy"]) < 1e-6)
+        e = np.angle(np.exp(1j * (zs["phase"] - ref["phase"])))[same]
+        line["cpu_baseline"] = {
+            "value": round(k / nwy / cdt, 6), "unit": "fields/s", "cores": nthreads,
+            "kind": "port",
+            "sample": f"FP64 oracle on the same field over {k} of the {nwy} wy rows "
+                      f"({k * nwx} of {nwx * nwy} pairs, {cdt:.1f} s), scaled by {nwy}/{k}"}
+        line["parity"] = {"sample_pairs": k * nwx, "pair_mismatch_frac": float(1 - same.mean()),
+                          "r_rel_max": float(np.abs(zs["r"] - ref["r"]).max() / ref["r"].max()),
+                          "phase_rms_rad": float(np.sqrt(np.mean(e * e)))}
+    print(json.dumps(line), flush=True)
+
+
 def bench_batch(args, glsu, torch, np):
     """64 independent sequences of uneven sizes in one batched run per step
     (persistent launches, one graph replay per frame).  value = frames/s over
@@ -386,6 +477,8 @@ def main():
         return bench_learned(args, glsu, torch, np)
     if args.workload == "aia1024":
         return bench_aia(args, glsu, torch, np)
+    if args.workload == "wfr1024":
+        return bench_wfr(args, glsu, torch, np)
     size, T, wff, gen = WORKLOADS[args.workload]
     W = H = args.size or size
     T = args.frames or T

@@ -98,6 +98,15 @@ extern "C" int glsu_wff_params_default(glsu_wff_params* p) {
   return GLSU_OK;
 }
 
+extern "C" int glsu_wfr_params_default(glsu_wff_params* p) {
+  if (!p) return fail(GLSU_ERR_ARG, "null params");
+  // WFT/WFT2_CPU.cpp:56-67 (WFR-mode constructor defaults)
+  p->sigma_x = 10.0; p->wxl = -2; p->wxi = 0.025; p->wxh = 2;
+  p->sigma_y = 10.0; p->wyl = -2.0; p->wyi = 0.025; p->wyh = 2.0;
+  p->thr = -1;
+  return GLSU_OK;
+}
+
 extern "C" int glsu_wff_params_dpra_reference(glsu_wff_params* p) {
   if (!p) return fail(GLSU_ERR_ARG, "null params");
   // DPRA/dpra_cpu.cpp:23-25: float literals widened to double
@@ -202,9 +211,65 @@ static int plan_common(WffPlan& p, int W, int H, const glsu_wff_params* pr, cons
   return GLSU_OK;
 }
 
+// WFR mode (WFT/WFT2_CPU.cpp:848-858): the constructor also fails when the
+// curvature padding OPT(H + 3 sy) or OPT(W + 3 sx) exceeds 4096.  The library
+// evaluates the curvatures directly, with LDS tiles that bound the window.
+static int validate_wfr(int W, int H, const glsu_wff_params* pr) {
+  const int sx = (int)std::round(3 * pr->sigma_x), sy = (int)std::round(3 * pr->sigma_y);
+  if (W + 3 * sx > 4096 || H + 3 * sy > 4096)
+    return fail(GLSU_ERR_PADDING,
+                "FFT padding is out of range [4096]. Shrink the size of either the image or the "
+                "Gaussian Window!");
+  if (wfr_curv_lds_bytes(sx, sy) > 65536)
+    return fail(GLSU_ERR_UNSUPPORTED, "WFR2 curvature window too large (sigma %g x %g)",
+                pr->sigma_x, pr->sigma_y);
+  const int nwy = (int)((pr->wyh - pr->wyl) * (1 / pr->wyi)) + 1;
+  if (nwy >= 0xFFFF) return fail(GLSU_ERR_UNSUPPORTED, "WFR2 supports fewer than 65535 wy values");
+  return GLSU_OK;
+}
+
+static int wfr_alloc(const WffPlan& p, WfrBufs& w) {
+  HIP_TRY(hipMalloc((void**)&w.d_idx, sizeof(uint16_t) * p.plane_elems * p.nwx));
+  std::vector<float> gx(2 * p.sx + 1), gy(2 * p.sy + 1);
+  double sgx = 0, sgy = 0, sxx = 0, syy = 0;
+  for (int k = -p.sx; k <= p.sx; ++k) {
+    const double g = std::exp(-(double)k * k / 2 / p.sigx / p.sigx);
+    gx[k + p.sx] = (float)g;
+    sgx += g;
+    sxx += (double)k * k * g;
+  }
+  for (int k = -p.sy; k <= p.sy; ++k) {
+    const double g = std::exp(-(double)k * k / 2 / p.sigy / p.sigy);
+    gy[k + p.sy] = (float)g;
+    sgy += g;
+    syy += (double)k * k * g;
+  }
+  HIP_TRY(hipMalloc((void**)&w.d_gx, sizeof(float) * gx.size()));
+  HIP_TRY(hipMalloc((void**)&w.d_gy, sizeof(float) * gy.size()));
+  HIP_TRY(hipMemcpy(w.d_gx, gx.data(), sizeof(float) * gx.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(w.d_gy, gy.data(), sizeof(float) * gy.size(), hipMemcpyHostToDevice));
+  w.c.inv_sx = (float)(1.0 / (sgy * sxx));
+  w.c.inv_sy = (float)(1.0 / (sgx * syy));
+  w.c.sig2x = (float)(p.sigx * p.sigx);
+  w.c.sig2y = (float)(p.sigy * p.sigy);
+  w.c.q4x = (float)(0.25 / M_PI / (p.sigx * p.sigx));
+  w.c.q4y = (float)(0.25 / M_PI / (p.sigy * p.sigy));
+  return GLSU_OK;
+}
+
+static void wfr_free(WfrBufs& w) {
+  hipFree(w.d_idx);
+  hipFree(w.d_gx);
+  hipFree(w.d_gy);
+  w = WfrBufs{};
+}
+
 // =============================================================== WFT2
 struct glsu_wft2 {
   WffPlan p;
+  int type = GLSU_WFF;
+  WfrBufs wfr;              // WFR mode only
+  float* d_wfr_out = nullptr;  // WFR host form: staging for the eight maps
   hipStream_t s = nullptr;
   bool own = false;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -217,17 +282,25 @@ extern "C" int glsu_wft2_create_tuned(glsu_wft2** out, int W, int H, int type,
                                       const glsu_tuning* t) {
   if (!out || !pr) return fail(GLSU_ERR_ARG, "null argument");
   *out = nullptr;
-  if (type == GLSU_WFR) return fail(GLSU_ERR_UNSUPPORTED, "WFR2 mode is not provided");
-  if (type != GLSU_WFF) return fail(GLSU_ERR_ARG, "unknown WFT type %d", type);
+  if (type != GLSU_WFF && type != GLSU_WFR) return fail(GLSU_ERR_ARG, "unknown WFT type %d", type);
   {
     int px = t ? t->planx : -1, py = t ? t->plany : -1;
     if (int rc = validate(W, H, pr, &px, &py)) return rc;
     if (int rc = validate_tuning(t)) return rc;
+    if (type == GLSU_WFR)
+      if (int rc = validate_wfr(W, H, pr)) return rc;
   }
   if (int rc = check_device()) return rc;
   auto* h = new glsu_wft2();
+  h->type = type;
   int rc = plan_common(h->p, W, H, pr, t);
   if (rc) { delete h; return rc; }
+  if (type == GLSU_WFR && (rc = wfr_alloc(h->p, h->wfr)) != GLSU_OK) {
+    wfr_free(h->wfr);
+    wff_plan_free(h->p);
+    delete h;
+    return rc;
+  }
   rc = make_stream(stream, &h->s, &h->own);
   if (rc) { wff_plan_free(h->p); delete h; return rc; }
   hipEventCreate(&h->e0);
@@ -254,6 +327,8 @@ static int wft2_enqueue(glsu_wft2* h, const float2* f, float2* out) {
 
 extern "C" int glsu_wft2_compute(glsu_wft2* h, const void* d_f, void* d_out, double* ms) {
   if (!h || !d_f || !d_out) return fail(GLSU_ERR_ARG, "null argument");
+  if (h->type != GLSU_WFF)
+    return fail(GLSU_ERR_ARG, "this handle is in WFR mode: use glsu_wft2_compute_wfr");
   if (ms) HIP_TRY(hipEventRecord(h->e0, h->s));
   if (int rc = wft2_enqueue(h, (const float2*)d_f, (float2*)d_out)) return rc;
   if (ms) {
@@ -268,6 +343,8 @@ extern "C" int glsu_wft2_compute(glsu_wft2* h, const void* d_f, void* d_out, dou
 
 extern "C" int glsu_wft2_compute_host(glsu_wft2* h, const float* hf, float* hout, double* ms) {
   if (!h || !hf || !hout) return fail(GLSU_ERR_ARG, "null argument");
+  if (h->type != GLSU_WFF)
+    return fail(GLSU_ERR_ARG, "this handle is in WFR mode: use glsu_wft2_compute_wfr_host");
   const size_t bytes = sizeof(float2) * (size_t)h->p.W * h->p.H;
   if (!h->d_in) {
     HIP_TRY(hipMalloc((void**)&h->d_in, bytes));
@@ -284,6 +361,102 @@ extern "C" int glsu_wft2_compute_host(glsu_wft2* h, const float* hf, float* hout
     HIP_TRY(hipEventElapsedTime(&t, h->e0, h->e1));
     *ms = t;
   }
+  return GLSU_OK;
+}
+
+// ---- WFR mode ---------------------------------------------------------------
+static int wfr_enqueue(glsu_wft2* h, const float2* f, const WfrOut& z) {
+  HIP_TRY(launch_rows_fwd(h->p, f, h->s));
+  HIP_TRY(launch_wfr(h->p, h->wfr, z, h->s));
+  return GLSU_OK;
+}
+
+static int wfr_out_from(const glsu_wfr_results* r, WfrOut* z) {
+  if (!r || !r->wx || !r->wy || !r->phase || !r->phase_comp || !r->b || !r->r || !r->cxx ||
+      !r->cyy)
+    return fail(GLSU_ERR_ARG, "all eight WFR result pointers are required");
+  *z = WfrOut{r->wx, r->wy, r->phase, r->phase_comp, r->b, r->r, r->cxx, r->cyy};
+  return GLSU_OK;
+}
+
+extern "C" int glsu_wft2_compute_wfr(glsu_wft2* h, const void* d_f, const glsu_wfr_results* z,
+                                     double* ms) {
+  if (!h || !d_f) return fail(GLSU_ERR_ARG, "null argument");
+  if (h->type != GLSU_WFR) return fail(GLSU_ERR_ARG, "this handle is in WFF mode");
+  WfrOut o;
+  if (int rc = wfr_out_from(z, &o)) return rc;
+  if (ms) HIP_TRY(hipEventRecord(h->e0, h->s));
+  if (int rc = wfr_enqueue(h, (const float2*)d_f, o)) return rc;
+  if (ms) {
+    HIP_TRY(hipEventRecord(h->e1, h->s));
+    HIP_TRY(hipEventSynchronize(h->e1));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, h->e0, h->e1));
+    *ms = t;
+  }
+  return GLSU_OK;
+}
+
+extern "C" int glsu_wft2_compute_wfr_host(glsu_wft2* h, const float* hf, float* hout,
+                                          double* ms) {
+  if (!h || !hf || !hout) return fail(GLSU_ERR_ARG, "null argument");
+  if (h->type != GLSU_WFR) return fail(GLSU_ERR_ARG, "this handle is in WFF mode");
+  const size_t n = (size_t)h->p.W * h->p.H;
+  if (!h->d_in) HIP_TRY(hipMalloc((void**)&h->d_in, sizeof(float2) * n));
+  if (!h->d_wfr_out) HIP_TRY(hipMalloc((void**)&h->d_wfr_out, sizeof(float) * 8 * n));
+  float* b = h->d_wfr_out;
+  const WfrOut o{b, b + n, b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n, b + 6 * n, b + 7 * n};
+  if (ms) HIP_TRY(hipEventRecord(h->e0, h->s));
+  HIP_TRY(hipMemcpyAsync(h->d_in, hf, sizeof(float2) * n, hipMemcpyHostToDevice, h->s));
+  if (int rc = wfr_enqueue(h, h->d_in, o)) return rc;
+  HIP_TRY(hipMemcpyAsync(hout, b, sizeof(float) * 8 * n, hipMemcpyDeviceToHost, h->s));
+  if (ms) HIP_TRY(hipEventRecord(h->e1, h->s));
+  HIP_TRY(hipStreamSynchronize(h->s));
+  if (ms) {
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, h->e0, h->e1));
+    *ms = t;
+  }
+  return GLSU_OK;
+}
+
+// Average time of one stage over reps back-to-back launches on the handle's
+// stream, after one full transform of d_f (which leaves the intermediate
+// planes in place): 1 the row kernel, 2 the column kernel (WFF or WFR), 3 the
+// rest (WFF: the inverse rows; WFR: ridge + curvature, writing into scratch
+// maps).  No reference counterpart (measurement).
+extern "C" int glsu_wft2_time_kernel(glsu_wft2* h, const void* d_f, int stage, int reps,
+                                     double* ms) {
+  if (!h || !d_f || !ms || reps < 1 || stage < 1 || stage > 3)
+    return fail(GLSU_ERR_ARG, "invalid argument");
+  const float2* f = (const float2*)d_f;
+  const size_t n = (size_t)h->p.W * h->p.H;
+  float2* out = nullptr;
+  float* maps = nullptr;
+  WfrOut z{};
+  if (h->type == GLSU_WFF) {
+    HIP_TRY(hipMalloc((void**)&out, sizeof(float2) * n));
+    if (int rc = wft2_enqueue(h, f, out)) { hipFree(out); return rc; }
+  } else {
+    HIP_TRY(hipMalloc((void**)&maps, sizeof(float) * 8 * n));
+    z = WfrOut{maps, maps + n, maps + 2 * n, maps + 3 * n, maps + 4 * n, maps + 5 * n,
+               maps + 6 * n, maps + 7 * n};
+    if (int rc = wfr_enqueue(h, f, z)) { hipFree(maps); return rc; }
+  }
+  hipError_t e = hipEventRecord(h->e0, h->s);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) {
+    if (stage == 1) e = launch_rows_fwd(h->p, f, h->s);
+    else if (stage == 2) e = h->type == GLSU_WFF ? launch_cols(h->p, h->s) : launch_wfr_cols(h->p, h->wfr, h->s);
+    else e = h->type == GLSU_WFF ? launch_rows_inv_filtered(h->p, out, h->s) : launch_wfr_post(h->p, h->wfr, z, h->s);
+  }
+  if (e == hipSuccess) e = hipEventRecord(h->e1, h->s);
+  if (e == hipSuccess) e = hipEventSynchronize(h->e1);
+  float t = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, h->e0, h->e1);
+  hipFree(out);
+  hipFree(maps);
+  if (e != hipSuccess) return fail(GLSU_ERR_DEVICE, "timing failed: %s", hipGetErrorString(e));
+  *ms = t / reps;
   return GLSU_OK;
 }
 
@@ -325,6 +498,8 @@ extern "C" int glsu_wft2_destroy(glsu_wft2* h) {
   if (!h) return GLSU_OK;
   hipStreamSynchronize(h->s);
   wff_plan_free(h->p);
+  wfr_free(h->wfr);
+  hipFree(h->d_wfr_out);
   hipFree(h->d_in);
   hipFree(h->d_out);
   hipEventDestroy(h->e0);

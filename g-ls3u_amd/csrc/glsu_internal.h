@@ -42,6 +42,27 @@ struct WffPlan {
   size_t plane_elems = 0;   // elements per tiled plane
 };
 
+// WFR mode: the eight result maps (device, row-major H x W floats), the
+// per-wx ridge index planes and the curvature taps and constants.
+struct WfrOut {
+  float *wx, *wy, *phase, *phase_comp, *b, *r, *cxx, *cyy;
+};
+struct WfrConst {
+  float inv_sx, inv_sy;  // 1 / (sum gy * sum k^2 gx), 1 / (sum gx * sum k^2 gy)
+  float sig2x, sig2y;    // sigma^2
+  float q4x, q4y;        // 1 / (4 pi sigma^2)
+};
+struct WfrBufs {
+  uint16_t* d_idx = nullptr;  // [nwx] tiled planes: wy index of the best pair
+  float* d_gx = nullptr;      // exp(-k^2 / 2 sigx^2), k = -sx..sx
+  float* d_gy = nullptr;
+  WfrConst c{};
+};
+hipError_t launch_wfr(const WffPlan& p, const WfrBufs& w, const WfrOut& z, hipStream_t s);
+hipError_t launch_wfr_cols(const WffPlan& p, const WfrBufs& w, hipStream_t s);  // D, idx planes
+hipError_t launch_wfr_post(const WffPlan& p, const WfrBufs& w, const WfrOut& z, hipStream_t s);
+size_t wfr_curv_lds_bytes(int sx, int sy);
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2;
 // remap so that each XCD walks a contiguous range of work items (lines shared
 // by neighbouring items stay in one L2).

@@ -42,6 +42,12 @@ class NotPositiveDefiniteError(GlsuError):
     that and exits (AIA/aia_cpuf.cpp:163-167)."""
 
 
+class WfrResults(ctypes.Structure):
+    """glsu_wfr_results: device pointers of the eight WFR maps."""
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ("wx", "wy", "phase", "phase_comp", "b", "r", "cxx", "cyy")]
+
+
 class Tuning(ctypes.Structure):
     """glsu_tuning: plan ids and launch shapes (-1 = library default)."""
     _fields_ = [("planx", ctypes.c_int), ("plany", ctypes.c_int),
@@ -109,6 +115,7 @@ def lib():
         "glsu_version": (ctypes.c_char_p, []),
         "glsu_device_count": (I, [PI]),
         "glsu_wff_params_default": (I, [PP]),
+        "glsu_wfr_params_default": (I, [PP]),
         "glsu_wff_params_dpra_reference": (I, [PP]),
         "glsu_wft2_create": (I, [PV, I, I, I, PP, V]),
         "glsu_wft2_create_tuned": (I, [PV, I, I, I, PP, V, PT]),
@@ -117,6 +124,9 @@ def lib():
         "glsu_wft2_compute_host": (I, [V, PF, PF, PD]),
         "glsu_wft2_info": (I, [V, PI, PI, PI, PI, PI, PI, PD]),
         "glsu_wft2_destroy": (I, [V]),
+        "glsu_wft2_compute_wfr": (I, [V, V, ctypes.POINTER(WfrResults), PD]),
+        "glsu_wft2_compute_wfr_host": (I, [V, V, V, PD]),
+        "glsu_wft2_time_kernel": (I, [V, V, I, I, PD]),
         "glsu_dpra_create": (I, [PV, PD, I, I, I, PP, V]),
         "glsu_dpra_create_f32": (I, [PV, PF, I, I, I, PP, V]),
         "glsu_dpra_create_tuned": (I, [PV, PD, I, I, I, PP, V, PT]),
@@ -188,10 +198,11 @@ def check(rc):
 def exported_symbols():
     """Names declared in include/glsu.h (for the load/export test)."""
     return [
-        "glsu_last_error", "glsu_version", "glsu_device_count", "glsu_wff_params_default",
+        "glsu_last_error", "glsu_version", "glsu_device_count", "glsu_wff_params_default", "glsu_wfr_params_default",
         "glsu_wff_params_dpra_reference", "glsu_wft2_create", "glsu_wft2_create_tuned",
         "glsu_wft2_compute", "glsu_wft2_compute_host", "glsu_wft2_info", "glsu_wft2_get_tuning",
-        "glsu_wft2_destroy",
+        "glsu_wft2_destroy", "glsu_wft2_compute_wfr", "glsu_wft2_compute_wfr_host",
+        "glsu_wft2_time_kernel",
         "glsu_dpra_create", "glsu_dpra_create_f32", "glsu_dpra_create_tuned", "glsu_dpra_get_tuning",
         "glsu_dpra_frame",
         "glsu_dpra_frame_host", "glsu_dpra_update_ref", "glsu_dpra_run", "glsu_dpra_run_graph",

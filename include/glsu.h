@@ -27,7 +27,7 @@ extern "C" {
 #define GLSU_ERR_PADDING (-2)     /* padded FFT size out of range (ref: throw -1)    */
 #define GLSU_ERR_SIZE (-3)        /* frame size differs from the initialised size    */
 #define GLSU_ERR_DEVICE (-4)      /* HIP runtime error                               */
-#define GLSU_ERR_UNSUPPORTED (-5) /* mode not provided by this library (WFR2)        */
+#define GLSU_ERR_UNSUPPORTED (-5) /* configuration outside this library's limits      */
 #define GLSU_ERR_NOT_PD (-6)      /* AIA normal matrix not positive definite (ref:
                                      prints and exits, AIA/aia_cpuf.cpp:163-167)    */
 
@@ -71,6 +71,9 @@ int glsu_device_count(int *count);
 /* Defaults of the WFF-mode constructor WFT2_cpu(W, H, WFF, z, n),
  * WFT/WFT2_CPU.cpp:43-54 (sigma 10, [-2-3/sigma, 2+3/sigma] step 1/sigma, thr<0). */
 int glsu_wff_params_default(glsu_wff_params *p);
+/* Defaults of the WFR-mode constructor, WFT/WFT2_CPU.cpp:56-67 (sigma 10,
+ * [-2, 2] step 0.025 on both axes; thr unused). */
+int glsu_wfr_params_default(glsu_wff_params *p);
 /* Parameters the reference DPRA constructors hard-code,
  * DPRA/dpra_cpu.cpp:23-25 and DPRA/dpra_cudaf.cu:116-118
  * (sigma 20, [-0.15f, 0.15f] step 0.05f, thr 15). */
@@ -82,7 +85,8 @@ int glsu_wff_params_dpra_reference(glsu_wff_params *p);
  * WFT/WFT2_CUDAf.h:27-33, WFT/WFT2_CUDAf.cu:1321-1419 (and the FP64 CPU twin
  * WFT2_cpu, WFT/WFT2_CPU.h:30-36).  Returns GLSU_ERR_PADDING where the
  * reference constructor throws int -1 (padded size above 4096,
- * WFT/WFT2_CPU.cpp:69-73), GLSU_ERR_UNSUPPORTED for type == GLSU_WFR. */
+ * WFT/WFT2_CPU.cpp:69-73).  type GLSU_WFR selects the ridge mode
+ * (glsu_wft2_compute_wfr). */
 int glsu_wft2_create(glsu_wft2 **out, int width, int height, int type,
                      const glsu_wff_params *p, void *hip_stream);
 /* Same, with explicit tuning (NULL = defaults); used by the autotuner.
@@ -104,6 +108,28 @@ int glsu_wft2_compute_host(glsu_wft2 *h, const float *h_f, float *h_filtered, do
 /* The plan ids and launch shapes in use (after defaults and LDS limits are
  * applied).  No counterpart in the reference. */
 int glsu_wft2_get_tuning(glsu_wft2 *h, glsu_tuning *t);
+
+/* WFR mode (type GLSU_WFR at create): the ridge maps of WFT2_DeviceResultsF
+ * (WFT/WFT.h:126-133), each height*width floats on the device. */
+typedef struct glsu_wfr_results {
+  float *wx, *wy, *phase, *phase_comp, *b, *r, *cxx, *cyy;
+} glsu_wfr_results;
+/* Replaces WFT2_CUDAF::operator() in WFR mode (cuWFR2, WFT/WFT2_CUDAf.cu:
+ * 1174-1319) with the semantics of WFT2_cpu::WFR2 (WFT/WFT2_CPU.cpp:391-694)
+ * and MatLab/fp_wft2f.m:103-147: per pixel the (wx, wy) pair of largest
+ * |f * w_p| (first in (wy, wx) order on exact ties) gives r, wx, wy and phase;
+ * cxx, cyy are the least-squares slopes of the border-replicated wx, wy maps;
+ * phase_comp and b the compensated phase and amplitude.  The constructor
+ * also fails (GLSU_ERR_PADDING) when W + 3 sx or H + 3 sy exceeds 4096
+ * (WFR2_Init, :848-858). */
+int glsu_wft2_compute_wfr(glsu_wft2 *h, const void *d_f, const glsu_wfr_results *z, double *ms);
+/* Host form: h_out receives the eight maps in the order of glsu_wfr_results
+ * (8 * height * width floats). */
+int glsu_wft2_compute_wfr_host(glsu_wft2 *h, const float *h_f, float *h_out, double *ms);
+/* Average device time of one kernel stage over reps launches, after one full
+ * transform of d_f: 1 rows, 2 columns, 3 the rest (WFF: inverse rows; WFR:
+ * ridge and curvature).  No reference counterpart (measurement). */
+int glsu_wft2_time_kernel(glsu_wft2 *h, const void *d_f, int stage, int reps, double *ms);
 int glsu_wft2_info(glsu_wft2 *h, int *nx, int *ny, int *sx, int *sy, int *nwx, int *nwy,
                    double *thr);
 int glsu_wft2_destroy(glsu_wft2 *h);

@@ -30,15 +30,21 @@ namespace WFT {
 
 enum class WFT_TYPE { WFF, WFR };  // Utils/WFT-FPA.h:72-76
 
-// Device results of the WFF mode.  As in the reference, the WFT2 constructor
-// allocates m_d_filtered (height*width complex64) and this struct frees it.
+// Device results.  As in the reference, the WFT2 constructor allocates the
+// members of its mode -- m_d_filtered (height*width complex64) in WFF mode,
+// the eight height*width float maps in WFR mode (WFT/WFT.h:114-135) -- and
+// this struct frees them.
 struct WFT2_DeviceResultsF {
   float2* m_d_filtered = nullptr;
+  float *m_d_wx = nullptr, *m_d_wy = nullptr, *m_d_phase = nullptr, *m_d_phase_comp = nullptr,
+        *m_d_b = nullptr, *m_d_r = nullptr, *m_d_cxx = nullptr, *m_d_cyy = nullptr;
   WFT2_DeviceResultsF() = default;
   WFT2_DeviceResultsF(const WFT2_DeviceResultsF&) = delete;
   WFT2_DeviceResultsF& operator=(const WFT2_DeviceResultsF&) = delete;
   ~WFT2_DeviceResultsF() {
     if (m_d_filtered) (void)hipFree(m_d_filtered);
+    for (float* q : {m_d_wx, m_d_wy, m_d_phase, m_d_phase_comp, m_d_b, m_d_r, m_d_cxx, m_d_cyy})
+      if (q) (void)hipFree(q);
   }
 };
 
@@ -53,18 +59,21 @@ class WFT2_HIPF {
   WFT2_HIPF(int iWidth, int iHeight, WFT_TYPE type, double rSigmaX, double rWxl, double rWxh,
             double rWxi, double rSigmaY, double rWyl, double rWyh, double rWyi, double rThr,
             WFT2_DeviceResultsF& z, int /*iNumStreams*/ = 1, hipStream_t stream = nullptr)
-      : w_(iWidth), h_(iHeight) {
+      : w_(iWidth), h_(iHeight), type_(type) {
     glsu_wff_params p{rSigmaX, rWxl, rWxh, rWxi, rSigmaY, rWyl, rWyh, rWyi, rThr};
     glsu_detail::check(glsu_wft2_create(&h, iWidth, iHeight,
                                         type == WFT_TYPE::WFF ? GLSU_WFF : GLSU_WFR, &p, stream));
     alloc(z);
   }
-  // Defaults of the WFF-mode constructor (WFT/WFT2_CPU.cpp:43-54).
+  // Defaults of the type's constructor (WFT/WFT2_CPU.cpp:43-67).
   WFT2_HIPF(int iWidth, int iHeight, WFT_TYPE type, WFT2_DeviceResultsF& z, int = 1,
             hipStream_t stream = nullptr)
-      : w_(iWidth), h_(iHeight) {
+      : w_(iWidth), h_(iHeight), type_(type) {
     glsu_wff_params p;
-    glsu_wff_params_default(&p);
+    if (type == WFT_TYPE::WFF)
+      glsu_wff_params_default(&p);
+    else
+      glsu_wfr_params_default(&p);
     glsu_detail::check(glsu_wft2_create(&h, iWidth, iHeight,
                                         type == WFT_TYPE::WFF ? GLSU_WFF : GLSU_WFR, &p, stream));
     alloc(z);
@@ -74,17 +83,31 @@ class WFT2_HIPF {
   // operator()(d_f, d_z, time), WFT/WFT2_CUDAf.h:38-40; d_f: device complex64.
   // time: milliseconds of device time (the call synchronises, as the reference).
   void operator()(const float2* d_f, WFT2_DeviceResultsF& d_z, double& time) {
-    glsu_detail::check(glsu_wft2_compute(h, d_f, d_z.m_d_filtered, &time));
+    if (type_ == WFT_TYPE::WFF) {
+      glsu_detail::check(glsu_wft2_compute(h, d_f, d_z.m_d_filtered, &time));
+    } else {
+      const glsu_wfr_results r{d_z.m_d_wx, d_z.m_d_wy, d_z.m_d_phase, d_z.m_d_phase_comp,
+                               d_z.m_d_b,  d_z.m_d_r,  d_z.m_d_cxx,   d_z.m_d_cyy};
+      glsu_detail::check(glsu_wft2_compute_wfr(h, d_f, &r, &time));
+    }
   }
 
  private:
   void alloc(WFT2_DeviceResultsF& z) {
-    if (!z.m_d_filtered &&
-        hipMalloc((void**)&z.m_d_filtered, sizeof(float2) * (size_t)w_ * h_) != hipSuccess)
-      throw std::runtime_error("glsu: allocation of the filtered field failed");
+    const size_t n = (size_t)w_ * h_;
+    if (type_ == WFT_TYPE::WFF) {
+      if (!z.m_d_filtered && hipMalloc((void**)&z.m_d_filtered, sizeof(float2) * n) != hipSuccess)
+        throw std::runtime_error("glsu: allocation of the filtered field failed");
+      return;
+    }
+    for (float** q : {&z.m_d_wx, &z.m_d_wy, &z.m_d_phase, &z.m_d_phase_comp, &z.m_d_b, &z.m_d_r,
+                      &z.m_d_cxx, &z.m_d_cyy})
+      if (!*q && hipMalloc((void**)q, sizeof(float) * n) != hipSuccess)
+        throw std::runtime_error("glsu: allocation of the ridge maps failed");
   }
   glsu_wft2* h = nullptr;
   int w_, h_;
+  WFT_TYPE type_;
 };
 
 }  // namespace WFT

@@ -932,3 +932,193 @@ out:
     free(old);
     return info;
 }
+
+/* ========================================================================== */
+/* WFR2: windowed Fourier ridges (the WFR mode of the same WFT2 classes).      */
+/* ========================================================================== */
+
+/* WFR2, WFT/WFT2_CPU.cpp:391-694 and MatLab/fp_wft2f.m:103-147, in FP64.
+ * out: 8 planes of W*H doubles -- wx, wy, phase, phase_comp, b, r, cxx, cyy
+ * (the members of WFT2_HostResults, WFT/WFT.h:52-60).
+ *   ridge   for each pair (wy outer, wx inner) sf = f * gwave (the same
+ *           convolution as WFF2), and where |sf| > r: r = |sf|, wx, wy and
+ *           phase = angle(sf) are taken from that pair.  Strict > as in
+ *           fp_wft2f.m:126 and WFT2_CUDAf.cu:637 (the C++ CPU class uses >=,
+ *           WFT2_CPU.cpp:505, which differs only on exact ties: here the first
+ *           pair in (wy, wx) order wins, and a pixel whose |sf| is 0 for every
+ *           pair keeps r = wx = wy = phase = 0).
+ *   curv    cxx = -conv2(wx_rep, x.*g)/sum(x.*x.*g), cyy likewise with y, where
+ *           wx_rep replicates the border pixels by (sy, sx) (:540-628,
+ *           fp_wft2f.m:133-141).  Evaluated directly and separably (g = gx gy,
+ *           the normalisation cancels); the reference evaluates it by FFT over
+ *           OPT(H+3sy) x OPT(W+3sx), which never wraps at the sampled points.
+ *   comp    phase_comp = wrap(phase - atan(sx^2 cxx)/2 - atan(sy^2 cyy)/2),
+ *           b = r ((1 + sigx^4 cxx^2)/(4 pi sigx^2))^(1/4) (same in y) (:674-691).
+ * Returns -1 when OPT(H + 3sy) or OPT(W + 3sx) does not exist (WFR2_Init,
+ * :848-858, where the reference constructor fails). */
+int orc_wfr2_compute(orc_wft2 *w, const double *f, double *out) {
+    const int W = w->W, H = w->H, PW = w->PW, PH = w->PH, sx = w->sx, sy = w->sy;
+    const size_t np = (size_t)PH * PW, ni = (size_t)W * H;
+    if (orc_first_greater(H + sy + 2 * sy) < 0 || orc_first_greater(W + sx + 2 * sx) < 0)
+        return -1;
+    /* feed (no threshold in WFR mode), Ff = fft2(fpad), :396-400 */
+    for (int i = 0; i < H; i++)
+        for (int j = 0; j < W; j++) {
+            size_t idf = (size_t)i * W + j, idp = (size_t)i * PW + j;
+            w->fpad[2 * idp] = f[2 * idf];
+            w->fpad[2 * idp + 1] = f[2 * idf + 1];
+        }
+    memcpy(w->Ff, w->fpad, sizeof(double) * 2 * np);
+    fft2_exec(&w->ctx[0], w->Ff, -1);
+    const int iwx = w->nwx, iwy = w->nwy, T = w->nthreads;
+    const double inv_np = 1.0 / (double)(PH * PW);
+    /* per-thread ridge maps: r, pair order, sf (re, im) */
+    double *rr = (double *)calloc(ni * T, sizeof(double));
+    int *ord = (int *)malloc(sizeof(int) * ni * T);
+    double *sre = (double *)calloc(ni * T, sizeof(double));
+    double *sim = (double *)calloc(ni * T, sizeof(double));
+    for (size_t i = 0; i < ni * T; i++) ord[i] = -1;
+#pragma omp parallel num_threads(T)
+    {
+        int tid = 0, nth = 1;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+        nth = omp_get_num_threads();
+#endif
+        double *gw = w->gwave + 2 * np * tid;
+        double *Fg = w->Fg + 2 * np * tid;
+        double *Sf = w->Sf + 2 * np * tid;
+        double *Gx = w->Gx + 2 * (size_t)PW * tid;
+        double *Gy = w->Gy + 2 * (size_t)PH * tid;
+        fft2ctx *ctx = &w->ctx[tid];
+        double *r_t = rr + ni * tid, *re_t = sre + ni * tid, *im_t = sim + ni * tid;
+        int *o_t = ord + ni * tid;
+        for (int y = tid; y < iwy; y += nth) {
+            double wyt = w->wyl + (double)y * w->wyi;
+            if (w->flags & ORC_FG_SEPARABLE)
+                spectrum_1d(&w->py[tid], w->gyn, sy, wyt, Gy, ctx->work);
+            for (int x = 0; x < iwx; x++) {
+                double wxt = w->wxl + (double)x * w->wxi;
+                if (w->flags & ORC_FG_SEPARABLE) {
+                    spectrum_1d(&w->px[tid], w->gxn, sx, wxt, Gx, ctx->work);
+                    for (int i = 0; i < PH; i++)
+                        for (int j = 0; j < PW; j++) {
+                            size_t id = (size_t)i * PW + j;
+                            double ar = Gy[2 * i], ai = Gy[2 * i + 1];
+                            double br = Gx[2 * j], bi = Gx[2 * j + 1];
+                            Fg[2 * id] = ar * br - ai * bi;
+                            Fg[2 * id + 1] = ar * bi + ai * br;
+                        }
+                } else {
+                    /* gwave = g .* exp(j(wxt*xx + wyt*yy)), :455-474 */
+                    memset(gw, 0, sizeof(double) * 2 * np);
+                    for (int i = 0; i < 2 * sy + 1; i++)
+                        for (int j = 0; j < 2 * sx + 1; j++) {
+                            size_t id = (size_t)i * PW + j;
+                            double yy = i - sy, xx = j - sx;
+                            double c = cos(wxt * xx + wyt * yy), s = sin(wxt * xx + wyt * yy);
+                            double g = w->gpad[2 * id];
+                            gw[2 * id] = g * c;
+                            gw[2 * id + 1] = g * s;
+                        }
+                    memcpy(Fg, gw, sizeof(double) * 2 * np);
+                    fft2_exec(ctx, Fg, -1);
+                }
+                /* sf = ifft2(Ff .* Fg), :476-493 */
+                for (size_t id = 0; id < np; id++) {
+                    double ar = w->Ff[2 * id], ai = w->Ff[2 * id + 1];
+                    double br = Fg[2 * id], bi = Fg[2 * id + 1];
+                    Sf[2 * id] = (ar * br - ai * bi) * inv_np;
+                    Sf[2 * id + 1] = (ar * bi + ai * br) * inv_np;
+                }
+                fft2_exec(ctx, Sf, +1);
+                /* ridge update on the (+sy, +sx) crop, :495-518 */
+                const int order = y * iwx + x;
+                for (int i = 0; i < H; i++)
+                    for (int j = 0; j < W; j++) {
+                        size_t idt = (size_t)(i + sy) * PW + j + sx, id = (size_t)i * W + j;
+                        double re = Sf[2 * idt], im = Sf[2 * idt + 1];
+                        double a = sqrt(re * re + im * im);
+                        if (a > r_t[id]) {
+                            r_t[id] = a;
+                            o_t[id] = order;
+                            re_t[id] = re;
+                            im_t[id] = im;
+                        }
+                    }
+            }
+        }
+    }
+    /* merge (:523-540): largest r, ties to the earlier pair in (wy, wx) order */
+    double *owx = out, *owy = out + ni, *oph = out + 2 * ni, *opc = out + 3 * ni;
+    double *ob = out + 4 * ni, *orr = out + 5 * ni, *ocx = out + 6 * ni, *ocy = out + 7 * ni;
+    for (size_t id = 0; id < ni; id++) {
+        double best = 0;
+        int bo = -1;
+        double bre = 0, bim = 0;
+        for (int t = 0; t < T; t++) {
+            size_t k = ni * t + id;
+            if (ord[k] >= 0 && (rr[k] > best || (rr[k] == best && bo >= 0 && ord[k] < bo))) {
+                best = rr[k]; bo = ord[k]; bre = sre[k]; bim = sim[k];
+            }
+        }
+        orr[id] = best;
+        if (bo >= 0) {
+            owx[id] = w->wxl + (double)(bo % iwx) * w->wxi;
+            owy[id] = w->wyl + (double)(bo / iwx) * w->wyi;
+            oph[id] = atan2(bim, bre);
+        } else {
+            owx[id] = owy[id] = oph[id] = 0;
+        }
+    }
+    free(rr); free(ord); free(sre); free(sim);
+    /* curvature: separable direct convolution with replicated borders */
+    double *gx = (double *)malloc(sizeof(double) * (2 * sx + 1));
+    double *gy = (double *)malloc(sizeof(double) * (2 * sy + 1));
+    double sgx = 0, sgy = 0, sxxgx = 0, syygy = 0;
+    for (int k = -sx; k <= sx; k++) {
+        gx[k + sx] = exp(-(double)(k * k) / 2 / w->sigx / w->sigx);
+        sgx += gx[k + sx];
+        sxxgx += (double)k * k * gx[k + sx];
+    }
+    for (int k = -sy; k <= sy; k++) {
+        gy[k + sy] = exp(-(double)(k * k) / 2 / w->sigy / w->sigy);
+        sgy += gy[k + sy];
+        syygy += (double)k * k * gy[k + sy];
+    }
+    double *rx = (double *)malloc(sizeof(double) * ni), *ry = (double *)malloc(sizeof(double) * ni);
+#pragma omp parallel for num_threads(T) schedule(static)
+    for (int i = 0; i < H; i++)
+        for (int j = 0; j < W; j++) {
+            double ax = 0, ay = 0;
+            for (int k = -sx; k <= sx; k++) {
+                int jj = j - k < 0 ? 0 : (j - k >= W ? W - 1 : j - k);
+                ax += (double)k * gx[k + sx] * owx[(size_t)i * W + jj];
+                ay += gx[k + sx] * owy[(size_t)i * W + jj];
+            }
+            rx[(size_t)i * W + j] = ax;
+            ry[(size_t)i * W + j] = ay;
+        }
+    const double sgx2 = w->sigx * w->sigx, sgy2 = w->sigy * w->sigy;
+#pragma omp parallel for num_threads(T) schedule(static)
+    for (int i = 0; i < H; i++)
+        for (int j = 0; j < W; j++) {
+            double cx = 0, cy = 0;
+            for (int k = -sy; k <= sy; k++) {
+                int ii = i - k < 0 ? 0 : (i - k >= H ? H - 1 : i - k);
+                cx += gy[k + sy] * rx[(size_t)ii * W + j];
+                cy += (double)k * gy[k + sy] * ry[(size_t)ii * W + j];
+            }
+            size_t id = (size_t)i * W + j;
+            cx = -cx / (sgy * sxxgx);
+            cy = -cy / (sgx * syygy);
+            ocx[id] = cx;
+            ocy[id] = cy;
+            double pc = oph[id] - 0.5 * atan(sgx2 * cx) - 0.5 * atan(sgy2 * cy);
+            opc[id] = atan2(sin(pc), cos(pc));
+            ob[id] = orr[id] * pow((1 + sgx2 * sgx2 * cx * cx) * 0.25 / M_PI / sgx2, 0.25) *
+                     pow((1 + sgy2 * sgy2 * cy * cy) * 0.25 / M_PI / sgy2, 0.25);
+        }
+    free(rx); free(ry); free(gx); free(gy);
+    return 0;
+}

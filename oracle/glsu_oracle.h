@@ -58,6 +58,9 @@ int    orc_wft2_pairs(const orc_wft2 *w, int *nwy, int *nwx);
 double orc_wft2_thr(const orc_wft2 *w);
 /* f, out: H*W interleaved complex doubles.  Returns 0 or -1. */
 int    orc_wft2_compute(orc_wft2 *w, const double *f, double *out);
+/* WFR mode: out = 8 planes of W*H doubles (wx, wy, phase, phase_comp, b, r,
+ * cxx, cyy).  Returns 0, or -1 when the curvature padding is out of range. */
+int    orc_wfr2_compute(orc_wft2 *w, const double *f, double *out);
 
 /* ---- DPRA (DPRA_CPU) ------------------------------------------------------ */
 typedef struct orc_dpra orc_dpra;

@@ -51,6 +51,7 @@ def lib():
             "orc_wft2_pairs": (I, [V, _ip, _ip]),
             "orc_wft2_thr": (D, [V]),
             "orc_wft2_compute": (I, [V, _dp, _dp]),
+            "orc_wfr2_compute": (I, [V, _dp, _dp]),
             "orc_dpra_create": (V, [_dp, I, I, I, D, D, D, D, D, D, D, D, D, I, I]),
             "orc_dpra_destroy": (None, [V]),
             "orc_dpra_ls": (I, [V, _dp, _dp]),
@@ -200,6 +201,23 @@ class WFF2:
         lib().orc_wft2_compute(self._h, f.view(np.float64).ctypes.data_as(_dp),
                                out.view(np.float64).ctypes.data_as(_dp))
         return out
+
+
+WFR_FIELDS = ("wx", "wy", "phase", "phase_comp", "b", "r", "cxx", "cyy")
+
+
+class WFR2(WFF2):
+    """FP64 WFR2 (WFT2_cpu in WFR mode): returns a dict of the eight result
+    maps (WFT/WFT.h:52-60)."""
+
+    def __call__(self, f):
+        f = _c128(f)
+        assert f.shape == (self.H, self.W)
+        out = np.empty((8, self.H, self.W))
+        rc = lib().orc_wfr2_compute(self._h, f.view(np.float64).ctypes.data_as(_dp), _ptr(out))
+        if rc:
+            raise ValueError("curvature padding is out of range [4096]")
+        return dict(zip(WFR_FIELDS, out))
 
 
 class DPRA:

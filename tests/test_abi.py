@@ -66,7 +66,10 @@ def test_padding_error_code():
 
 def test_argument_errors():
     assert _create_wft2(0, 10)[0] == _lib.ERR_ARG
-    assert _create_wft2(10, 10, typ=1)[0] == _lib.ERR_UNSUPPORTED      # WFR2 not provided
+    # WFR mode: the curvature padding rule (W + 3 sx <= 4096, WFT/WFT2_CPU.cpp:848-858)
+    # fails where WFF mode would pass, and a window beyond the curvature tiles
+    assert _create_wft2(4020, 16, typ=1)[0] == _lib.ERR_PADDING
+    assert _create_wft2(10, 10, typ=1, params=glsu.wff_params(sigma_x=40.0))[0] == _lib.ERR_UNSUPPORTED
     assert _create_wft2(10, 10, typ=7)[0] == _lib.ERR_ARG
     assert _create_wft2(10, 10, params=glsu.wff_params(wxi=0.0))[0] == _lib.ERR_ARG
     h = ctypes.c_void_p()

@@ -12,5 +12,6 @@ from .wft2 import WFT2  # noqa: F401
 from .dpra import DPRA, gen_frames  # noqa: F401
 from .batch import DPRABatch  # noqa: F401
 from .aia import AIA  # noqa: F401
+from .output import phase_to_u8  # noqa: F401
 from . import synth, autotune, learned, io  # noqa: F401
 from .learned import UNet  # noqa: F401

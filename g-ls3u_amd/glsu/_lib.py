@@ -169,6 +169,7 @@ def lib():
         "glsu_read_complex_text": (I, [ctypes.c_char_p, PI, PI, V, ctypes.c_size_t]),
         "glsu_write_complex_text": (I, [ctypes.c_char_p, I, I, V, I]),
         "glsu_fft_plans": (I, [PI, PI, PI, PI, I, PI]),
+        "glsu_phase_to_u8": (I, [V, V, ctypes.c_size_t, V]),
         "glsu_aia_create": (I, [PV, I, I, I, V]),
         "glsu_aia_run": (I, [V, V, I, V, I, U64, I, D, I, V, PI, PD, PD]),
         "glsu_aia_run_host": (I, [V, PV, I, V, I, U64, I, D, I, V, PI, PD, PD]),
@@ -214,7 +215,7 @@ def exported_symbols():
         "glsu_unet_loss_grad", "glsu_unet_adam_step", "glsu_unet_destroy",
         "glsu_dpra_frame_learned", "glsu_dpra_copy_net", "glsu_dpra_learned_stats",
         "glsu_read_matrix_bin", "glsu_write_matrix_bin", "glsu_read_complex_text",
-        "glsu_write_complex_text", "glsu_fft_plans",
+        "glsu_write_complex_text", "glsu_fft_plans", "glsu_phase_to_u8",
         "glsu_aia_create", "glsu_aia_run", "glsu_aia_run_host", "glsu_aia_info", "glsu_aia_time_iter", "glsu_aia_destroy",
     ]
 

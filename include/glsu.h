@@ -330,6 +330,13 @@ int glsu_read_complex_text(const char *path, int *rows, int *cols, double *data,
 int glsu_write_complex_text(const char *path, int rows, int cols, const double *data,
                             int precision);
 
+/* ---- output stage ----------------------------------------------------------
+ * Phase (e.g. the cumulative dphi of the chain, device floats) to 8-bit video
+ * frames as the reference demo writes them (App_DPRA/dprawidget.cpp:541-549):
+ * uchar((atan2(sin p, cos p) + pi) / 2 / pi * 255) in FP32, truncated.  n
+ * pixels (any number of frames), asynchronous on hip_stream (NULL: default). */
+int glsu_phase_to_u8(const float *d_phi, uint8_t *d_out, size_t n, void *hip_stream);
+
 /* ---- utilities ---------------------------------------------------------- */
 /* Seeded synthetic fringe frames, written to device memory (float32,
  * nframes*height*width): I = 128 + 100 cos(phi) + noise*N(0,1),

@@ -3,6 +3,7 @@
 //   WFT_FPA::WFT::WFT2_CUDAF  (WFT/WFT2_CUDAf.h:11-40)   -> WFT_FPA::WFT::WFT2_HIPF
 //   WFT_FPA::WFT::WFT2_DeviceResultsF (WFT/WFT.h:114-135) -> same name (WFF member)
 //   DPRA::DPRA_CUDAF          (DPRA/dpra_cudaf.h:21-45)   -> DPRA::DPRA_HIPF
+//   DPRA::DPRA_HYBRIDF        (DPRA/dpra_hybridf.h:14-40) -> DPRA::DPRA_HYBRID_HIPF
 //   AIA::AIA_CUDAF            (AIA/aia_cudaf.h:15-72)     -> AIA::AIA_HIPF
 // Error behaviour follows the reference: constructors throw int -1 when the
 // padded FFT size is out of range (WFT/WFT2_CPU.cpp:69-73); other failures,
@@ -176,7 +177,29 @@ class DPRA_HIPF {
   int rr_;
 };
 
+// DPRA::DPRA_HYBRIDF (DPRA/dpra_hybridf.h:14-40, dpra_hybridf.cpp:182-299)
+// builds the 3x3 systems on the GPU, copies them to the host, solves them with
+// LAPACKE_ssysv on iNumThreads OpenMP threads and copies the field back for
+// WFF2 on the GPU.  Same algorithm and parameters as DPRA_CUDAF; here every
+// step stays on the device (the host round trip per frame buys nothing on this
+// part), so the class is the GPU path under the hybrid's constructor.
+class DPRA_HYBRID_HIPF : public DPRA_HIPF {
+ public:
+  DPRA_HYBRID_HIPF(const float* v_Phi0, int iWidth, int iHeight, int irefUpdateRate = 1,
+                   int /*iNumThreads*/ = 1, hipStream_t stream = nullptr)
+      : DPRA_HIPF(v_Phi0, iWidth, iHeight, irefUpdateRate, stream) {}
+};
+
 }  // namespace DPRA
+
+namespace glsu_output {
+// Phase to 8-bit video frames on the device (App_DPRA/dprawidget.cpp:541-549):
+// n pixels of d_phi (float) to d_out (uint8), asynchronous on `stream`.
+inline void phase_to_u8(const float* d_phi, std::uint8_t* d_out, size_t n,
+                        hipStream_t stream = nullptr) {
+  glsu_detail::check(glsu_phase_to_u8(d_phi, d_out, n, stream));
+}
+}  // namespace glsu_output
 
 namespace AIA {
 

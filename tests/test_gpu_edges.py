@@ -109,8 +109,18 @@ def test_cpp_wrapper_example_runs():
     # "centre dphi <v> (model <m>)": the recovered phase change follows the model
     toks = r.stdout.split()
     v = float(toks[toks.index("dphi") + 1])
-    m = float(toks[toks.index("(model") + 1].rstrip(")"))
+    m = float(toks[toks.index("(model") + 1].rstrip(");"))
     assert abs(v - m) < 0.05
+    # the same loop through DPRA_HYBRID_HIPF (the DPRA_HYBRIDF constructor:
+    # the reference's WFF parameters, sigma 20, +-0.15 step 0.05, thr 15)
+    r = subprocess.run([exe, "256", "192", "6", "hybrid"], capture_output=True, text=True,
+                       timeout=120)
+    say(r.stdout.strip())
+    assert r.returncode == 0, r.stdout + r.stderr
+    toks = r.stdout.split()
+    v = float(toks[toks.index("dphi") + 1])
+    m = float(toks[toks.index("(model") + 1].rstrip(");"))
+    assert "hybrid" in r.stdout and abs(v - m) < 0.05
     # the constructor's padding error is the reference's int -1 throw
     r = subprocess.run([exe, "4096", "16", "1"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "threw -1" in r.stdout

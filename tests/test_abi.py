@@ -124,8 +124,8 @@ def test_no_device_is_reported_not_faked():
 
 def test_cpp_wrapper_compiles():
     # include/glsu.hpp (reference-style C++ classes) compiles against the ABI
-    src = os.path.join(ROOT, "g-ls3u_amd", "csrc", "example_dpra.cpp")
-    out = "/tmp/glsu_example_syntax.o"
-    r = subprocess.run(["/opt/rocm/bin/hipcc", "-std=c++17", "-fsyntax-only", src],
-                       capture_output=True, text=True)
-    assert r.returncode == 0, r.stderr[-2000:]
+    for name in ("example_dpra.cpp", "example_classes.cpp"):
+        src = os.path.join(ROOT, "g-ls3u_amd", "csrc", name)
+        r = subprocess.run(["/opt/rocm/bin/hipcc", "-std=c++17", "-fsyntax-only", src],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, name + ": " + r.stderr[-2000:]

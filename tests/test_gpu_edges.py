@@ -124,3 +124,16 @@ def test_cpp_wrapper_example_runs():
     # the constructor's padding error is the reference's int -1 throw
     r = subprocess.run([exe, "4096", "16", "1"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "threw -1" in r.stdout
+
+
+def test_cpp_aia_and_wfr_classes_example_runs():
+    # AIA::AIA_HIPF on six shifted 8-bit frames and WFT2_HIPF in WFR mode on a
+    # plane wave, from C++: the program checks the recovered shifts, phase,
+    # ridge frequency and zero curvature itself and exits 0 when they hold
+    exe = os.path.join(ROOT, "g-ls3u_amd", "lib", "example_classes")
+    if not os.path.exists(exe):
+        pytest.skip("example_classes not built")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    say(r.stdout.strip())
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "example_classes: ok" in r.stdout

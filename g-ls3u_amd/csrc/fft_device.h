@@ -287,7 +287,7 @@ struct FftPlan {
   static_assert(((E_ % R == 0) && ...), "every cross-lane radix must divide E");
   static_assert((R * ... * 1) == T_, "cross-lane radices must multiply to T");
   // spectral layout: k of register j in lane (c, g) = kbase(lane) + koff(j)
-  __device__ __forceinline__ static int kbase(int lane) {
+  __host__ __device__ __forceinline__ static int kbase(int lane) {
     const int c = lane % (T_ / RL), g = lane / (T_ / RL);
     return g + E_ * c;
   }

@@ -31,8 +31,9 @@ struct WffPlan {
   int Nx = 0, Ny = 0;
   int cols_waves = 4, rows_waves = 1;  // launch shapes
   // device buffers
-  float2* d_Hx = nullptr;   // [nwx][Nx], window spectra / Nx
-  float2* d_Hy = nullptr;   // [nwy][Ny], window spectra / Ny
+  // window spectra / N, lane-major: [w][j][lane] = bin kbase(lane) + koff(j)
+  float2* d_Hx = nullptr;   // [nwx][Nx]
+  float2* d_Hy = nullptr;   // [nwy][Ny]
   float2* d_twx = nullptr;  // W_Nx^k
   float2* d_twy = nullptr;  // W_Ny^k
   float2* d_A = nullptr;    // [nwx] tiled planes: row-convolved field

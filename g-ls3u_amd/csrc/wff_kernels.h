@@ -64,7 +64,6 @@ __device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ frow,
   float2* buf = smem + wave * P::LDS;
   FftLane<P> ctx;
   ctx.init(lane, tw_g);
-  const int kb = P::kbase(lane);
   float2 z[E];
   if (wave == 0) {
 #pragma unroll
@@ -83,12 +82,10 @@ __device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ frow,
   }
   __syncthreads();  // wave 0 reuses its buffer below
   for (int k = wave; k < nwx; k += RW) {
-    const float2* hx = Hx + (size_t)k * N + kb;
+    const float2* hx = Hx + (size_t)k * N + lane;  // lane-major: one 512-byte row per j
     float2 v[E];
-    sfor<0, E>([&](auto jc) {
-      constexpr int j = decltype(jc)::value;
-      v[j] = cmul(z[j], hx[P::koff(j)]);
-    });
+#pragma unroll
+    for (int j = 0; j < E; ++j) v[j] = cmul(z[j], hx[j * T]);
     fft_inv<P>(v, buf, ctx);
     float2* out = A + plane * k;
 #pragma unroll
@@ -129,7 +126,6 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
   const float thr2 = thr * thr;
   Ctx ctx;
   ctx.init(lane, tw_g, ttab);
-  const int kb = P::kbase(lane);
   const float2* colA = A + plane * k;
   float2 ah[E], acc[E];
 #pragma unroll
@@ -140,12 +136,10 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
   }
   fft_fwd<P>(ah, buf, ctx);
   for (int q = 0; q < nwy; ++q) {
-    const float2* hy = Hy + (size_t)q * N + kb;
+    const float2* hy = Hy + (size_t)q * N + lane;
     float2 v[E];
-    sfor<0, E>([&](auto jc) {
-      constexpr int j = decltype(jc)::value;
-      v[j] = cmul(ah[j], hy[P::koff(j)]);
-    });
+#pragma unroll
+    for (int j = 0; j < E; ++j) v[j] = cmul(ah[j], hy[j * T]);
     fft_inv<P>(v, buf, ctx);
     bool live = false;
 #pragma unroll
@@ -163,15 +157,15 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
     fft_fwd<P>(v, buf, ctx);
     // re-read the window spectrum instead of keeping 2E VGPRs live across the
     // two transforms: the opaque copy of the lane offset defeats load CSE
-    int kb2 = kb;
-    __asm__ volatile("" : "+v"(kb2));
-    const float2* hy2 = Hy + (size_t)q * N + kb2;
-    sfor<0, E>([&](auto jc) {
-      constexpr int j = decltype(jc)::value;
-      const float2 h = hy2[P::koff(j)];
+    int l2 = lane;
+    __asm__ volatile("" : "+v"(l2));
+    const float2* hy2 = Hy + (size_t)q * N + l2;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const float2 h = hy2[j * T];
       acc[j].x += v[j].x * h.x - v[j].y * h.y;
       acc[j].y += v[j].x * h.y + v[j].y * h.x;
-    });
+    }
   }
   fft_inv<P>(acc, buf, ctx);
   float2* colD = D + plane * k;
@@ -226,7 +220,6 @@ __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ D,
   float2* buf = smem + wave * P::LDS;
   FftLane<P> ctx;
   ctx.init(lane, tw_g);
-  const int kb = P::kbase(lane);
   float2 acc[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) acc[j] = make_float2(0.f, 0.f);
@@ -239,13 +232,13 @@ __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ D,
       v[j] = x < W ? Dk[tile_index(y, x, H)] : make_float2(0.f, 0.f);
     }
     fft_fwd<P>(v, buf, ctx);
-    const float2* hx = Hx + (size_t)k * N + kb;
-    sfor<0, E>([&](auto jc) {
-      constexpr int j = decltype(jc)::value;
-      const float2 h = hx[P::koff(j)];
+    const float2* hx = Hx + (size_t)k * N + lane;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const float2 h = hx[j * T];
       acc[j].x += v[j].x * h.x - v[j].y * h.y;
       acc[j].y += v[j].x * h.y + v[j].y * h.x;
-    });
+    }
   }
   if (RW > 1) {
     if (wave != 0) {

@@ -123,6 +123,32 @@ static const FftPlanInfo* find_plan(int id) {
   return nullptr;
 }
 
+// Window spectra in the order the kernels read them: entry (r, j, lane) holds
+// spectral bin kbase(lane) + koff(j) of row r, so that register j of every
+// lane of a transform is one contiguous T-element run (a coalesced load).
+template <class P>
+static void to_lane_major(std::vector<float2>& h, int rows) {
+  std::vector<float2> out(h.size());
+  for (int r = 0; r < rows; ++r)
+    for (int j = 0; j < P::E; ++j)
+      for (int l = 0; l < P::T; ++l)
+        out[(size_t)r * P::N + j * P::T + l] = h[(size_t)r * P::N + P::kbase(l) + P::koff(j)];
+  h.swap(out);
+}
+
+static bool lane_major(int plan, std::vector<float2>& h, int rows) {
+  switch (plan) {
+#define BODY                \
+  to_lane_major<P>(h, rows); \
+  return true
+    GLSU_PLANS(GLSU_DISPATCH_CASE)
+#undef BODY
+    default:
+      break;
+  }
+  return false;
+}
+
 // (1/N) * DFT of the centred, modulated 1-D window g1(d) e^{j w d}, d in [-s, s]:
 // H[k] = (1/N) sum_d g1(d) e^{j w d} e^{-2 pi i k d / N}
 static std::vector<float2> window_spectrum(const std::vector<double>& g1, int s, double w, int N) {
@@ -231,6 +257,10 @@ int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh,
   for (int k = 0; k < p.nwy; ++k) {
     auto t = window_spectrum(gy, p.sy, wyl + (double)k * wyi, p.Ny);
     hy.insert(hy.end(), t.begin(), t.end());
+  }
+  if (!lane_major(planx, hx, p.nwx) || !lane_major(plany, hy, p.nwy)) {
+    err = "unknown FFT plan";
+    return -2;
   }
   hipError_t e;
   if ((e = upload(p.d_Hx, hx)) != hipSuccess || (e = upload(p.d_Hy, hy)) != hipSuccess ||

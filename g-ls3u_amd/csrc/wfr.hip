@@ -38,7 +38,6 @@ __device__ __forceinline__ void cols_wfr_body(const float2* __restrict__ A,
   using Ctx = FftLane<P, true>;
   Ctx ctx;
   ctx.init(lane, tw_g, ttab);
-  const int kb = P::kbase(lane);
   const float2* colA = A + plane * k;
   float2 ah[E], best[E];
   int bidx[E];
@@ -51,12 +50,10 @@ __device__ __forceinline__ void cols_wfr_body(const float2* __restrict__ A,
   }
   fft_fwd<P>(ah, buf, ctx);
   for (int q = 0; q < nwy; ++q) {
-    const float2* hy = Hy + (size_t)q * N + kb;
+    const float2* hy = Hy + (size_t)q * N + lane;  // lane-major window spectra
     float2 v[E];
-    sfor<0, E>([&](auto jc) {
-      constexpr int j = decltype(jc)::value;
-      v[j] = cmul(ah[j], hy[P::koff(j)]);
-    });
+#pragma unroll
+    for (int j = 0; j < E; ++j) v[j] = cmul(ah[j], hy[j * T]);
     fft_inv<P>(v, buf, ctx);
 #pragma unroll
     for (int j = 0; j < E; ++j) {

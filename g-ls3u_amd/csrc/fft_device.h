@@ -319,12 +319,23 @@ struct FftLane {
       ttab_lds[i] = twN[(l * k) % N];
     }
   }
-  __device__ __forceinline__ float2 tw_trans(int k) const {
+  // Transition twiddles of one transform: tw_row() once per transform, then
+  // tw_trans(row, k) for each k.  With TT_LDS the row pointer goes through an
+  // opaque copy of the lane index, so every transform re-reads the table (the
+  // compiler cannot keep E twiddle pairs live across transforms) while the E
+  // reads of one transform stay independent and issue back to back.
+  __device__ __forceinline__ const float2* tw_row() const {
     if constexpr (TT_LDS) {
-      // opaque lane offset: re-read per transform rather than cache in VGPRs
       int o = lane;
       __asm__ volatile("" : "+v"(o));
-      return ttab[k * T + o];
+      return ttab + o;
+    } else {
+      return nullptr;
+    }
+  }
+  __device__ __forceinline__ float2 tw_trans(const float2* row, int k) const {
+    if constexpr (TT_LDS) {
+      return row[k * T];
     } else {
       return tt[k];
     }
@@ -435,9 +446,10 @@ __device__ __forceinline__ void fft_fwd(float2 (&v)[P::E], float2* buf, const Ct
   constexpr int E = P::E, ROW = P::ROW;
   Dft<E, false>::run(v);
   float2* wb = buf + P::phys(L_.lane);
+  const float2* tr = L_.tw_row();
   sfor<0, E>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    wb[k * ROW] = (k == 0) ? v[k] : cmul(v[k], L_.tw_trans(k));
+    wb[k * ROW] = (k == 0) ? v[k] : cmul(v[k], L_.tw_trans(tr, k));
   });
   wave_sync();
   sfor<0, P::NS>([&](auto sc) { stage_fwd<P, decltype(sc)::value, false>(v, buf, L_); });
@@ -449,10 +461,11 @@ __device__ __forceinline__ void fft_inv(float2 (&v)[P::E], float2* buf, const Ct
   constexpr int E = P::E, ROW = P::ROW;
   stages_inv<P, P::NS - 1>(v, buf, L_);
   const float2* rb = buf + P::phys(L_.lane);
+  const float2* tr = L_.tw_row();
   sfor<0, E>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     const float2 a = rb[k * ROW];
-    v[k] = (k == 0) ? a : cmulc(a, L_.tw_trans(k));
+    v[k] = (k == 0) ? a : cmulc(a, L_.tw_trans(tr, k));
   });
   Dft<E, true>::run(v);
 }

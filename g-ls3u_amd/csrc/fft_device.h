@@ -444,12 +444,24 @@ __device__ __forceinline__ void stages_inv(float2 (&v)[P::E], float2* buf, const
 template <class P, class Ctx>
 __device__ __forceinline__ void fft_fwd(float2 (&v)[P::E], float2* buf, const Ctx& L_) {
   constexpr int E = P::E, ROW = P::ROW;
+  // LDS twiddles: issue the table reads before the in-register DFT, whose
+  // arithmetic then covers their latency (the scheduling barrier keeps the
+  // scheduler from sinking them back to their uses)
+  float2 tw[E];
+  const float2* tr = L_.tw_row();
+  if constexpr (Ctx::kTTLds) {
+    sfor<1, E>([&](auto kc) { tw[decltype(kc)::value] = L_.tw_trans(tr, decltype(kc)::value); });
+    __builtin_amdgcn_sched_barrier(0);
+  }
   Dft<E, false>::run(v);
   float2* wb = buf + P::phys(L_.lane);
-  const float2* tr = L_.tw_row();
   sfor<0, E>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    wb[k * ROW] = (k == 0) ? v[k] : cmul(v[k], L_.tw_trans(tr, k));
+    if constexpr (k == 0) {
+      wb[0] = v[0];
+    } else {
+      wb[k * ROW] = cmul(v[k], Ctx::kTTLds ? tw[k] : L_.tw_trans(tr, k));
+    }
   });
   wave_sync();
   sfor<0, P::NS>([&](auto sc) { stage_fwd<P, decltype(sc)::value, false>(v, buf, L_); });

@@ -434,10 +434,10 @@ __device__ __forceinline__ void stage_inv(float2 (&v)[P::E], float2* buf, const 
   wave_sync();
 }
 
-template <class P, int S, class Ctx>
+template <class P, int S, class Ctx, int LO = 0>
 __device__ __forceinline__ void stages_inv(float2 (&v)[P::E], float2* buf, const Ctx& L_) {
   stage_inv<P, S>(v, buf, L_);
-  if constexpr (S > 0) stages_inv<P, S - 1>(v, buf, L_);
+  if constexpr (S > LO) stages_inv<P, S - 1, Ctx, LO>(v, buf, L_);
 }
 
 // Forward DFT (unnormalised): spatial layout in, spectral layout out.
@@ -471,13 +471,25 @@ __device__ __forceinline__ void fft_fwd(float2 (&v)[P::E], float2* buf, const Ct
 template <class P, class Ctx>
 __device__ __forceinline__ void fft_inv(float2 (&v)[P::E], float2* buf, const Ctx& L_) {
   constexpr int E = P::E, ROW = P::ROW;
-  stages_inv<P, P::NS - 1>(v, buf, L_);
-  const float2* rb = buf + P::phys(L_.lane);
+  if constexpr (P::NS > 1) stages_inv<P, P::NS - 1, Ctx, 1>(v, buf, L_);
+  // LDS twiddles: issued before the last stage, whose exchange and
+  // butterflies cover their latency
+  float2 tw[E];
   const float2* tr = L_.tw_row();
+  if constexpr (Ctx::kTTLds) {
+    sfor<1, E>([&](auto kc) { tw[decltype(kc)::value] = L_.tw_trans(tr, decltype(kc)::value); });
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  stage_inv<P, 0>(v, buf, L_);
+  const float2* rb = buf + P::phys(L_.lane);
   sfor<0, E>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     const float2 a = rb[k * ROW];
-    v[k] = (k == 0) ? a : cmulc(a, L_.tw_trans(tr, k));
+    if constexpr (k == 0) {
+      v[0] = a;
+    } else {
+      v[k] = cmulc(a, Ctx::kTTLds ? tw[k] : L_.tw_trans(tr, k));
+    }
   });
   Dft<E, true>::run(v);
 }

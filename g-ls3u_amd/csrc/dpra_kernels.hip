@@ -26,17 +26,33 @@ __device__ __forceinline__ void dpra_ls_body(const InT* __restrict__ I,
   __shared__ float sC[DTY + 2][DTX + 2];  // cos(phi/2)
   const int tx = threadIdx.x % DTX, ty0 = (threadIdx.x / DTX) * DRPT;
   const int x0 = bx * DTX - 1, y0 = by * DTY - 1;
-  for (int i = threadIdx.x; i < (DTX + 2) * (DTY + 2); i += DTHREADS) {
+  // Staging of the (DTX+2) x (DTY+2) tile with its halo: KS slots per thread.
+  // Every load is issued before any is used, from clamped (always in-bounds)
+  // coordinates, and the out-of-image values are replaced afterwards: loads
+  // under per-slot guards would each wait for the previous slot's round trip.
+  constexpr int NS = (DTX + 2) * (DTY + 2), KS = (NS + DTHREADS - 1) / DTHREADS;
+  float ld_i[KS], ld_p[KS];
+  bool in[KS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int i = min((int)threadIdx.x + k * DTHREADS, NS - 1);
     const int ly = i / (DTX + 2), lx = i % (DTX + 2);
     const int gx = x0 + lx, gy = y0 + ly;
-    float vi = 0.f, hs = 0.f, hc = 1.f;
-    if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-      vi = load_in(I, (size_t)gy * pitch + gx);
-      sincos_half(0.5f * phr[(size_t)gy * W + gx], &hs, &hc);
-    }
-    sI[ly][lx] = vi;
-    sS[ly][lx] = hs;
-    sC[ly][lx] = hc;
+    in[k] = gx >= 0 && gx < W && gy >= 0 && gy < H;
+    const int cx = min(max(gx, 0), W - 1), cy = min(max(gy, 0), H - 1);
+    ld_i[k] = load_in(I, (size_t)cy * pitch + cx);
+    ld_p[k] = phr[(size_t)cy * W + cx];
+  }
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int i = (int)threadIdx.x + k * DTHREADS;
+    if (i >= NS) break;
+    const int ly = i / (DTX + 2), lx = i % (DTX + 2);
+    float hs = 0.f, hc = 1.f;
+    sincos_half(0.5f * ld_p[k], &hs, &hc);
+    sI[ly][lx] = in[k] ? ld_i[k] : 0.f;
+    sS[ly][lx] = in[k] ? hs : 0.f;
+    sC[ly][lx] = in[k] ? hc : 1.f;
   }
   __syncthreads();
   const int x = bx * DTX + tx;

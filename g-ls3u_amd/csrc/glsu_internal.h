@@ -108,21 +108,34 @@ hipError_t launch_rows_inv_update(const WffPlan& p, const UpdateArgs& u, hipStre
 // (DPRA/dpra_cpu.cpp:244-251): phicurr = wrap(phiref + t) into (-pi, pi]
 // with 2 pi split in two floats; dphi = dphiref + t carried as an unevaluated
 // float pair (TwoSum, then Fast2Sum renormalisation).
-__device__ __forceinline__ void phase_update(const UpdateArgs& u, size_t id, float t) {
-  float pc = u.phiref[id] + t;
+// phase_step: the arithmetic on values already loaded (phiref, dphiref hi/lo);
+// phase_store writes its results.  Kernels that update several pixels per
+// thread load all inputs first: the output stores could alias the inputs, so
+// a load after a store is not moved above it, and on this target the wait for
+// that load also waits for the stores before it.
+struct PhaseOut {
+  float pc, nh, nl;
+};
+__device__ __forceinline__ PhaseOut phase_step(float pr, float hi, float lo, float t) {
+  float pc = pr + t;
   if (pc > 3.14159265f) pc = (pc - 6.28318548f) + 1.74845553e-7f;
   else if (pc <= -3.14159265f) pc = (pc + 6.28318548f) - 1.74845553e-7f;
-  u.phicurr[id] = pc;
-  const float hi = u.dref_hi[id], lo = u.dref_lo[id];
   const float s = hi + t;
   const float bp = s - hi;
   const float err = (hi - (s - bp)) + (t - bp);
   const float lo2 = lo + err;
   const float nh = s + lo2;
   const float nl = lo2 - (nh - s);
-  u.dphi_hi[id] = nh;
-  u.dphi_lo[id] = nl;
-  if (u.dphi_out) u.dphi_out[id] = nh;
+  return PhaseOut{pc, nh, nl};
+}
+__device__ __forceinline__ void phase_store(const UpdateArgs& u, size_t id, const PhaseOut& o) {
+  u.phicurr[id] = o.pc;
+  u.dphi_hi[id] = o.nh;
+  u.dphi_lo[id] = o.nl;
+  if (u.dphi_out) u.dphi_out[id] = o.nh;
+}
+__device__ __forceinline__ void phase_update(const UpdateArgs& u, size_t id, float t) {
+  phase_store(u, id, phase_step(u.phiref[id], u.dref_hi[id], u.dref_lo[id], t));
 }
 // learned path: update from the network output (t = angle(out)); residual
 // of the network's phase against a WFF2 window on a 64 x 64 tile

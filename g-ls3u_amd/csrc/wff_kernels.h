@@ -258,17 +258,32 @@ __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ D,
     }
   }
   fft_inv<P>(acc, buf, ctx);
+  if constexpr (MODE == 0) {
 #pragma unroll
-  for (int j = 0; j < E; ++j) {
-    const int x = lane + T * j;
-    if (x >= W) continue;
-    const size_t id = (size_t)y * W + x;
-    const float2 F = make_float2(acc[j].x * scale, acc[j].y * scale);
-    if constexpr (MODE == 0) {
-      filtered[id] = F;
-    } else {
+    for (int j = 0; j < E; ++j) {
+      const int x = lane + T * j;
+      if (x >= W) continue;
+      filtered[(size_t)y * W + x] = make_float2(acc[j].x * scale, acc[j].y * scale);
+    }
+  } else {
+    // every input of the update is loaded (clamped columns, always in the
+    // image) before the first output is stored
+    float pr[E], hi[E], lo[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const size_t id = (size_t)y * W + min(lane + T * j, W - 1);
+      pr[j] = u.phiref[id];
+      hi[j] = u.dref_hi[id];
+      lo[j] = u.dref_lo[id];
+    }
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const int x = lane + T * j;
+      if (x >= W) continue;
+      const size_t id = (size_t)y * W + x;
+      const float2 F = make_float2(acc[j].x * scale, acc[j].y * scale);
       if (u.filtered) u.filtered[id] = F;
-      phase_update(u, id, atan2f(F.y, F.x));
+      phase_store(u, id, phase_step(pr[j], hi[j], lo[j], atan2f(F.y, F.x)));
     }
   }
 }

@@ -16,9 +16,10 @@
 // The half-angle sines come from one sincos(phi/2) per pixel and the
 // difference formula sin(d/2) = sin(a/2)cos(b/2) - cos(a/2)sin(b/2), which has
 // no cancellation for small d, so e and s' keep full relative precision.  The
-// three sums whose terms change sign (s', e s', I s') are accumulated with a
-// compensated two-sum; the other five have terms of one sign and are plain
-// FP32 sums.  The DC unknown is eliminated (2x2 Schur complement) and the 2x2
+// three sums whose terms change sign (s', e s', I s') are accumulated in FP64
+// (exact for nine FP32 terms; FP64 adds and conversions issue at the FP32
+// rate on this target, cheaper than a compensated FP32 two-sum); the other
+// five have terms of one sign and are plain FP32 sums.  The DC unknown is eliminated (2x2 Schur complement) and the 2x2
 // system is solved by Cramer's rule; det(A) = 9 det2.  Where
 // |det A| < 1e-6 * A00*A11*A22 the 2x2 solve is redone in FP64 from the
 // compensated sums; where det A <= 1e-10 * A00*A11*A22 the pixel is singular
@@ -30,17 +31,12 @@
 
 namespace glsu {
 
-struct Comp {  // compensated FP32 accumulator (Knuth TwoSum)
-  float s, c;
-  __device__ __forceinline__ void init() { s = 0.f; c = 0.f; }
-  __device__ __forceinline__ void add(float x) {
-    const float t = s + x;
-    const float bp = t - s;
-    c += (s - (t - bp)) + (x - bp);
-    s = t;
-  }
-  __device__ __forceinline__ float f() const { return s + c; }
-  __device__ __forceinline__ double d() const { return (double)s + (double)c; }
+struct Comp {  // FP64 accumulator of FP32 terms (exact for these 9-term sums)
+  double s;
+  __device__ __forceinline__ void init() { s = 0.0; }
+  __device__ __forceinline__ void add(float x) { s += (double)x; }
+  __device__ __forceinline__ float f() const { return (float)s; }
+  __device__ __forceinline__ double d() const { return s; }
 };
 
 template <typename InT>

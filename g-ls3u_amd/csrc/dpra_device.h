@@ -31,7 +31,7 @@
 
 namespace glsu {
 
-struct Comp {  // FP64 accumulator of FP32 terms (exact for these 9-term sums)
+struct Acc64 {  // FP64 accumulator of FP32 terms (exact for these 9-term sums)
   double s;
   __device__ __forceinline__ void init() { s = 0.0; }
   __device__ __forceinline__ void add(float x) { s += (double)x; }
@@ -74,7 +74,7 @@ __device__ __forceinline__ float2 dpra_pixel(const float* sI, const float* sS, c
                                              int H) {
   const float hsp = sS[ly * pitch + lx], hcp = sC[ly * pitch + lx];
   float Se = 0.f, See = 0.f, Sss = 0.f, SI = 0.f, SIe = 0.f;
-  Comp Ss, Ses, SIs;
+  Acc64 Ss, Ses, SIs;
   Ss.init(); Ses.init(); SIs.init();
   int n = 0;
   auto neighbour = [&](int dy, int dx) {

@@ -27,7 +27,7 @@ for _p in (ROOT, os.path.join(ROOT, "g-ls3u_amd")):
 
 METRIC = "frames/s DPRA+WFF2 @1024²; learned vs plain at stated rad RMS vs FP64"
 PEAK_FP32_TFLOPS = 157.3
-# the same ALUs without packed FP32 instructions, which this build never emits
+# the same ALUs without packed FP32 instructions (the kernels are compiled without them)
 PEAK_FP32_UNPACKED_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -464,8 +464,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py: this build runs on exactly one GPU (the per-frame chain is "
-              "sequential; multi-GPU sharding is described in DESIGN.md, not built).",
+        print("bench.py: runs on exactly one GPU (the per-frame chain is sequential; "
+              "sharding across GPUs is not built).",
               file=sys.stderr)
         sys.exit(2)
 
@@ -539,7 +539,7 @@ def main():
         "kernel": "wff_cols_kernel", "avg_ms": round(col_ms, 5),
         "note": "FFT butterflies on the FP32 vector ALU (no dense contraction, so no MFMA); "
                 "priced against the FP32 peak, which MI355X's matrix and vector units share; "
-                "that peak assumes packed FP32, which this build does not emit (frac_unpacked "
+                "that peak assumes packed FP32, which the kernels are compiled without (frac_unpacked "
                 "is against the unpacked vector rate)",
         "frac_unpacked": round(achieved / PEAK_FP32_UNPACKED_TFLOPS, 4),
         "hbm_GBps": round(hbm, 1), "hbm_frac": round(hbm / PEAK_HBM_GBS, 4),

@@ -239,7 +239,7 @@ int glsu_dpra_batch_destroy(glsu_dpra_batch *h);
  * to the WFF2-filtered unit vector.  BF16 weights and activations on MFMA
  * with FP32 accumulation; FP32 master weights and Adam.  No counterpart in
  * the reference: it stands in for WFT2_cpu::operator() (WFT/WFT2_CPU.cpp:
- * 178-186) inside dpra_per_frame, and WFF2 stays its fallback and judge.
+ * 178-186) inside dpra_per_frame, and WFF2 stays its fallback and arbiter.
  * Parameters are flat FP32 in layer order (e0a e0b ... e{L-1}b d{L-2}a d{L-2}b
  * ... d0a d0b out), each layer's weights [Cout][k][k][Cin] then biases.
  * Images: n x height x width complex64, height a multiple of 4*2^(levels-1),

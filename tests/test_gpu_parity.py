@@ -1,6 +1,6 @@
 """GPU parity: libglsu (HIP, FP32) against the FP64 CPU oracle on the same inputs.
 
-Bars (north star): wrapped phase error of the DPRA chain <= 1e-3 rad RMS per
+Bars: wrapped phase error of the DPRA chain <= 1e-3 rad RMS per
 frame.  Isolated WFF2 on identical inputs: relative field error <= 1e-4 and
 phase error <= 1e-4 rad RMS.  DPRA least squares alone: <= 1e-4 rad RMS.
 """

@@ -38,6 +38,24 @@ __device__ __forceinline__ bf16x8 load8(const __bf16* __restrict__ X1, int C1,
   return *(const bf16x8*)(X2 + (((size_t)n * H2 + (y >> 1)) * W2 + (x >> 1)) * C2 + (c - C1));
 }
 
+// The same value as load8 without a branch around the load: the pixel is
+// clamped into the image, loaded unconditionally and replaced by zero when it
+// was outside, so that a thread's loads can all be in flight together.
+__device__ __forceinline__ bf16x8 load8_clamped(const __bf16* __restrict__ X1, int C1,
+                                                const __bf16* __restrict__ X2, int C2, int n,
+                                                int y, int x, int H, int W, int c) {
+  const bool in = y >= 0 && y < H && x >= 0 && x < W;
+  const int yc = min(max(y, 0), H - 1), xc = min(max(x, 0), W - 1);
+  bf16x8 v;
+  if (c < C1) {
+    v = *(const bf16x8*)(X1 + (((size_t)n * H + yc) * W + xc) * C1 + c);
+  } else {
+    const int H2 = H >> 1, W2 = W >> 1;
+    v = *(const bf16x8*)(X2 + (((size_t)n * H2 + (yc >> 1)) * W2 + (xc >> 1)) * C2 + (c - C1));
+  }
+  return in ? v : zero8();
+}
+
 // ------------------------------------------------------------------------
 // 3x3 convolution, implicit GEMM: M = pixels, N = output channels, K = 9*Cin.
 // Workgroup: 4 waves, output tile 4 rows x 64 pixels x NCO channels; wave w
@@ -111,12 +129,22 @@ __global__ __launch_bounds__(256) void conv3_mfma_kernel(
   for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
     bf16x8 b[NB];
     if (!kWLds) load_b(b, 0, ci0);
-    __syncthreads();  // previous chunk fully consumed
-    for (int i = threadIdx.x; i < kInPix * 4; i += 256) {
+    // the chunk's input tile: every 16-byte load of this thread is issued
+    // before the first is stored (one memory round trip, not one per load)
+    constexpr int kLd = (kInPix * 4 + 255) / 256;
+    bf16x8 ld[kLd];
+#pragma unroll
+    for (int r = 0; r < kLd; ++r) {
+      const int i = min((int)threadIdx.x + 256 * r, kInPix * 4 - 1);
       const int g = i & 3, pix = i >> 2;
       const int ry = pix / (kTileW + 2), rx = pix % (kTileW + 2);
-      *(bf16x8*)(sIn + swz(pix, g)) =
-          load8(X1, C1, X2, C2, img, y0 + ry - 1, x0 + rx - 1, H, W, ci0 + g * 8);
+      ld[r] = load8_clamped(X1, C1, X2, C2, img, y0 + ry - 1, x0 + rx - 1, H, W, ci0 + g * 8);
+    }
+    __syncthreads();  // previous chunk fully consumed
+#pragma unroll
+    for (int r = 0; r < kLd; ++r) {
+      const int i = (int)threadIdx.x + 256 * r;
+      if (i < kInPix * 4) *(bf16x8*)(sIn + swz(i >> 2, i & 3)) = ld[r];
     }
     if constexpr (kWLds) {
 #pragma unroll 1

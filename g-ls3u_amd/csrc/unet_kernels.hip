@@ -288,30 +288,33 @@ __global__ __launch_bounds__(256) void conv3_in_kernel(const float2* __restrict_
                                                        const float* __restrict__ b,
                                                        __bf16* __restrict__ Y, int n, int H,
                                                        int W, int C0) {
-  extern __shared__ float sw[];  // C0 * 18 weights then C0 biases
-  for (int i = threadIdx.x; i < C0 * 19; i += blockDim.x) sw[i] = i < C0 * 18 ? w[i] : b[i - C0 * 18];
-  __syncthreads();
+  // the weights and biases are read at wave-uniform addresses straight from
+  // global memory: the compiler turns them into scalar loads (SGPR operands of
+  // the FMAs) instead of one LDS read per multiply-add
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long P = (long)n * H * W;
   if (p >= P) return;
   const int xx = (int)(p % W), yy = (int)((p / W) % H), img = (int)(p / ((long)W * H));
+  // the nine taps are loaded from clamped (in-image) pixels without branches,
+  // so all nine loads are in flight together, then zeroed outside the image
   float in[18];
 #pragma unroll
   for (int t = 0; t < 9; ++t) {
     const int y2 = yy + t / 3 - 1, x2 = xx + t % 3 - 1;
-    float2 v = make_float2(0.f, 0.f);
-    if (y2 >= 0 && y2 < H && x2 >= 0 && x2 < W) v = x[((size_t)img * H + y2) * W + x2];
-    in[2 * t] = v.x;
-    in[2 * t + 1] = v.y;
+    const bool inside = y2 >= 0 && y2 < H && x2 >= 0 && x2 < W;
+    const int yc = min(max(y2, 0), H - 1), xc = min(max(x2, 0), W - 1);
+    const float2 v = x[((size_t)img * H + yc) * W + xc];
+    in[2 * t] = inside ? v.x : 0.f;
+    in[2 * t + 1] = inside ? v.y : 0.f;
   }
   for (int g = 0; g < C0 / 8; ++g) {
     bf16x8 o;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int co = g * 8 + k;
-      float s = sw[C0 * 18 + co];
+      float s = b[co];
 #pragma unroll
-      for (int i = 0; i < 18; ++i) s = fmaf(sw[co * 18 + i], in[i], s);
+      for (int i = 0; i < 18; ++i) s = fmaf(w[co * 18 + i], in[i], s);
       o[k] = (__bf16)fmaxf(s, 0.f);
     }
     *(bf16x8*)(Y + (size_t)p * C0 + g * 8) = o;
@@ -607,8 +610,8 @@ hipError_t unet_conv3_dw(const __bf16* dY, int Cout, const __bf16* X1, int C1, c
 hipError_t unet_conv3_in(const float2* x, const float* w, const float* b, __bf16* Y, int n,
                          int H, int W, int C0, hipStream_t s) {
   const long work = (long)n * H * W;
-  hipLaunchKernelGGL(conv3_in_kernel, dim3(blocks(work)), dim3(256), C0 * 19 * sizeof(float), s,
-                     x, w, b, Y, n, H, W, C0);
+  hipLaunchKernelGGL(conv3_in_kernel, dim3(blocks(work)), dim3(256), 0, s, x, w, b, Y, n, H, W,
+                     C0);
   return hipGetLastError();
 }
 

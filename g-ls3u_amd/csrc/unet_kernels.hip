@@ -126,9 +126,34 @@ __global__ __launch_bounds__(256) void conv3_mfma_kernel(
         acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb], b[nb], acc[mb][nb], 0, 0, 0);
   };
 
+  // weight staging (NCO >= 64): one row of three taps, 3 * NCO * 32 bf16, as
+  // kWl 16-byte loads per thread, all issued before any is stored; the next
+  // row's loads are in flight during the current row's MFMAs
+  constexpr int kWl = (3 * NCO * 4 + 255) / 256;
+  bf16x8 wl[kWLds ? kWl : 1];
+  auto load_w = [&](int tg, int ci0) {
+#pragma unroll
+    for (int r = 0; r < kWl; ++r) {
+      const int i = min((int)threadIdx.x + 256 * r, 3 * NCO * 4 - 1);
+      const int g = i & 3, rr = i >> 2;
+      const int tl = rr / NCO, co = rr % NCO;
+      const int coc = min(co0 + co, Cout - 1);
+      const bf16x8 v = *(const bf16x8*)(Wt + ((size_t)coc * 9 + 3 * tg + tl) * Cin + ci0 + g * 8);
+      wl[r] = co0 + co < Cout ? v : zero8();
+    }
+  };
+  auto store_w = [&]() {
+#pragma unroll
+    for (int r = 0; r < kWl; ++r) {
+      const int i = (int)threadIdx.x + 256 * r;
+      if (i < 3 * NCO * 4) *(bf16x8*)(sW + swz(i >> 2, i & 3)) = wl[r];
+    }
+  };
+
   for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
     bf16x8 b[NB];
     if (!kWLds) load_b(b, 0, ci0);
+    if constexpr (kWLds) load_w(0, ci0);
     // the chunk's input tile: every 16-byte load of this thread is issued
     // before the first is stored (one memory round trip, not one per load)
     constexpr int kLd = (kInPix * 4 + 255) / 256;
@@ -150,14 +175,8 @@ __global__ __launch_bounds__(256) void conv3_mfma_kernel(
 #pragma unroll 1
       for (int tg = 0; tg < 3; ++tg) {
         if (tg > 0) __syncthreads();  // previous tap row consumed
-        for (int i = threadIdx.x; i < 3 * NCO * 4; i += 256) {
-          const int g = i & 3, r = i >> 2;
-          const int tl = r / NCO, co = r % NCO;
-          bf16x8 v = zero8();
-          if (co0 + co < Cout)
-            v = *(const bf16x8*)(Wt + ((size_t)(co0 + co) * 9 + 3 * tg + tl) * Cin + ci0 + g * 8);
-          *(bf16x8*)(sW + swz(r, g)) = v;
-        }
+        store_w();
+        if (tg < 2) load_w(tg + 1, ci0);
         __syncthreads();
 #pragma unroll 1
         for (int tl = 0; tl < 3; ++tl) {

@@ -17,6 +17,7 @@
 namespace glsu {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 
 __device__ __forceinline__ bf16x8 zero8() {
@@ -119,11 +120,14 @@ __global__ __launch_bounds__(256) void conv3_mfma_kernel(
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb)
       a[mb] = *(const bf16x8*)(sIn + swz((wave + dy) * (kTileW + 2) + mb * 16 + lr + dx, lk));
+    // weights as the A operand, pixels as B: the 16 x 16 result is
+    // [output channel][pixel], so each lane ends with 4 consecutive channels
+    // of one pixel (one 8-byte NHWC store in the epilogue)
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb)
-        acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb], b[nb], acc[mb][nb], 0, 0, 0);
+        acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[nb], a[mb], acc[mb][nb], 0, 0, 0);
   };
 
   // weight staging (NCO >= 64): one row of three taps, 3 * NCO * 32 bf16, as
@@ -200,28 +204,36 @@ __global__ __launch_bounds__(256) void conv3_mfma_kernel(
       }
     }
   }
-  // epilogue: lane holds rows m = 4*lk + j (pixels) of column lr (channel)
+  // epilogue: lane holds rows 4*lk + j (output channels) of column lr (pixel)
   const int y = y0 + wave;
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
-    const int co = co0 + nb * 16 + lr;
-    if (co >= Cout) continue;
-    const float bv = (MODE == 0) ? bias[co] : 0.f;
+    const int cb = co0 + nb * 16 + 4 * lk;  // channels cb .. cb + 3 (Cout % 16 == 0)
+    if (cb >= Cout) continue;
+    float bv[4] = {0.f, 0.f, 0.f, 0.f};
+    if (MODE == 0)
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
+      for (int j = 0; j < 4; ++j) bv[j] = bias[cb + j];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      const int x = x0 + mb * 16 + lr;
+      if (x >= W) continue;
+      const size_t o = (((size_t)img * H + y) * W + x) * Cout + cb;
+      bf16x4 out;
+      bf16x4 mk;
+      if (MODE == 1 && mask) mk = *(const bf16x4*)(mask + o);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int x = x0 + mb * 16 + 4 * lk + j;
-        if (x >= W) continue;
-        const size_t o = (((size_t)img * H + y) * W + x) * Cout + co;
         float v = acc[mb][nb][j];
         if (MODE == 0) {
-          v = fmaxf(v + bv, 0.f);
+          v = fmaxf(v + bv[j], 0.f);
         } else if (mask) {
-          v = ((float)mask[o] > 0.f) ? v : 0.f;
+          v = ((float)mk[j] > 0.f) ? v : 0.f;
         }
-        Y[o] = (__bf16)v;
+        out[j] = (__bf16)v;
       }
+      *(bf16x4*)(Y + o) = out;
+    }
   }
 }
 
@@ -592,7 +604,8 @@ static unsigned blocks(long work, int per = 256) { return (unsigned)((work + per
 hipError_t unet_conv3(const __bf16* X1, int C1, const __bf16* X2, int C2, const __bf16* Wt,
                       const float* bias, const __bf16* mask, __bf16* Y, int n, int H, int W,
                       int Cout, int mode, hipStream_t s) {
-  if ((C1 % 32) || (C2 % 32) || (H % kTileH) || (C2 && ((H | W) & 1))) return hipErrorInvalidValue;
+  if ((C1 % 32) || (C2 % 32) || (Cout % 16) || (H % kTileH) || (C2 && ((H | W) & 1)))
+    return hipErrorInvalidValue;
   const int tiles = n * (H / kTileH) * ((W + kTileW - 1) / kTileW);
   const int nco = Cout >= 64 ? 64 : 32;
   dim3 grid(tiles, (Cout + nco - 1) / nco);

@@ -488,8 +488,9 @@ __global__ __launch_bounds__(kSolveThreads) void aia_solve_kernel(
       if (lane == 0) tk[k] = t;
     }
   const int nv = 5 + 2 * M;
-  // each wave reduces four values at a time; each lane keeps up to 32 loads in
-  // flight (four values x eight block strides), clamped in range and masked
+  // each wave reduces four values at a time; each lane keeps up to 64 loads in
+  // flight (four values x sixteen block strides: one round for up to 1024
+  // pass blocks), clamped in range and masked
   for (int v0 = wid * 4; v0 < nv; v0 += 4 * kWaves) {
     size_t base[4];
 #pragma unroll
@@ -507,16 +508,16 @@ __global__ __launch_bounds__(kSolveThreads) void aia_solve_kernel(
       base[j] = (size_t)row * Gtot + g0;
     }
     double s[4] = {0, 0, 0, 0};
-    for (int g = lane; g < G; g += 512) {
-      float t[8][4];
+    for (int g = lane; g < G; g += 1024) {
+      float t[16][4];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < 16; ++u) {
         const int gg = g + 64 * u < G ? g + 64 * u : g;
 #pragma unroll
         for (int j = 0; j < 4; ++j) t[u][j] = part[base[j] + gg];
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
+      for (int u = 0; u < 16; ++u)
         if (g + 64 * u < G)
 #pragma unroll
           for (int j = 0; j < 4; ++j) s[j] += (double)t[u][j];

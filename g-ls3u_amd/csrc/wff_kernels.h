@@ -135,12 +135,8 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
     acc[j] = make_float2(0.f, 0.f);
   }
   fft_fwd<P>(ah, buf, ctx);
-  for (int q = 0; q < nwy; ++q) {
-    const float2* hy = Hy + (size_t)q * N + lane;
-    float2 v[E];
-#pragma unroll
-    for (int j = 0; j < E; ++j) v[j] = cmul(ah[j], hy[j * T]);
-    fft_inv<P>(v, buf, ctx);
+  // threshold on image rows; false when no slot of the wavefront kept a sample
+  auto threshold = [&](float2(&v)[E]) {
     bool live = false;
 #pragma unroll
     for (int j = 0; j < E; ++j) {
@@ -150,21 +146,52 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
       if (!keep) v[j] = make_float2(0.f, 0.f);
       live |= keep;
     }
-    // a column whose thresholded signal is zero everywhere adds exactly
-    // nothing: skip its transform when no slot of the wavefront has a
-    // surviving sample (wave-uniform branch)
-    if (__builtin_amdgcn_ballot_w64(live) == 0) continue;
-    fft_fwd<P>(v, buf, ctx);
-    // re-read the window spectrum instead of keeping 2E VGPRs live across the
-    // two transforms: the opaque copy of the lane offset defeats load CSE
-    int l2 = lane;
-    __asm__ volatile("" : "+v"(l2));
-    const float2* hy2 = Hy + (size_t)q * N + l2;
+    return __builtin_amdgcn_ballot_w64(live) != 0;
+  };
+  if constexpr (E > 24) {
+    // long transforms run at one wavefront per SIMD, so nothing else hides an
+    // L2 round trip: the wy spectrum is read once and held across both of its
+    // products (the extra registers spill to AGPRs, cheaper than the stall)
+    for (int q = 0; q < nwy; ++q) {
+      const float2* hy = Hy + (size_t)q * N + lane;
+      float2 h[E], v[E];
 #pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const float2 h = hy2[j * T];
-      acc[j].x += v[j].x * h.x - v[j].y * h.y;
-      acc[j].y += v[j].x * h.y + v[j].y * h.x;
+      for (int j = 0; j < E; ++j) {
+        h[j] = hy[j * T];
+        v[j] = cmul(ah[j], h[j]);
+      }
+      fft_inv<P>(v, buf, ctx);
+      if (!threshold(v)) continue;
+      fft_fwd<P>(v, buf, ctx);
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        acc[j].x += v[j].x * h[j].x - v[j].y * h[j].y;
+        acc[j].y += v[j].x * h[j].y + v[j].y * h[j].x;
+      }
+    }
+  } else {
+    for (int q = 0; q < nwy; ++q) {
+      const float2* hy = Hy + (size_t)q * N + lane;
+      float2 v[E];
+#pragma unroll
+      for (int j = 0; j < E; ++j) v[j] = cmul(ah[j], hy[j * T]);
+      fft_inv<P>(v, buf, ctx);
+      // a column whose thresholded signal is zero everywhere adds exactly
+      // nothing: skip its transform when no slot of the wavefront has a
+      // surviving sample (wave-uniform branch)
+      if (!threshold(v)) continue;
+      fft_fwd<P>(v, buf, ctx);
+      // re-read the window spectrum instead of keeping 2E VGPRs live across
+      // the two transforms: the opaque copy of the lane offset defeats load CSE
+      int l2 = lane;
+      __asm__ volatile("" : "+v"(l2));
+      const float2* hy2 = Hy + (size_t)q * N + l2;
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const float2 hh = hy2[j * T];
+        acc[j].x += v[j].x * hh.x - v[j].y * hh.y;
+        acc[j].y += v[j].x * hh.y + v[j].y * hh.x;
+      }
     }
   }
   fft_inv<P>(acc, buf, ctx);

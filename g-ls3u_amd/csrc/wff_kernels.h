@@ -81,12 +81,31 @@ __device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ frow,
     for (int j = 0; j < E; ++j) z[j] = smem[j * T + lane];
   }
   __syncthreads();  // wave 0 reuses its buffer below
-  for (int k = wave; k < nwx; k += RW) {
+  // long transforms (one wavefront per SIMD): the next wx's spectrum is read
+  // before this wx's plane stores (the wait for a load also waits for every
+  // store issued before it); the extra registers spill to AGPRs
+  constexpr bool kPrefetch = E > 24;
+  float2 h[E];  // used only when kPrefetch
+  auto load_h = [&](int k) {
     const float2* hx = Hx + (size_t)k * N + lane;  // lane-major: one 512-byte row per j
-    float2 v[E];
 #pragma unroll
-    for (int j = 0; j < E; ++j) v[j] = cmul(z[j], hx[j * T]);
+    for (int j = 0; j < E; ++j) h[j] = hx[j * T];
+  };
+  if constexpr (kPrefetch)
+    if (wave < nwx) load_h(wave);
+  for (int k = wave; k < nwx; k += RW) {
+    float2 v[E];
+    if constexpr (kPrefetch) {
+#pragma unroll
+      for (int j = 0; j < E; ++j) v[j] = cmul(z[j], h[j]);
+    } else {
+      const float2* hx = Hx + (size_t)k * N + lane;
+#pragma unroll
+      for (int j = 0; j < E; ++j) v[j] = cmul(z[j], hx[j * T]);
+    }
     fft_inv<P>(v, buf, ctx);
+    if constexpr (kPrefetch)
+      if (k + RW < nwx) load_h(k + RW);
     float2* out = A + plane * k;
 #pragma unroll
     for (int j = 0; j < E; ++j) {
@@ -250,13 +269,28 @@ __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ D,
   float2 acc[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) acc[j] = make_float2(0.f, 0.f);
-  for (int k = wave; k < nwx; k += RW) {
+  auto load_row = [&](float2(&r)[E], int k) {
     const float2* Dk = D + plane * k;
-    float2 v[E];
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       const int x = lane + T * j;
-      v[j] = x < W ? Dk[tile_index(y, x, H)] : make_float2(0.f, 0.f);
+      r[j] = x < W ? Dk[tile_index(y, x, H)] : make_float2(0.f, 0.f);
+    }
+  };
+  // long transforms (one wavefront per SIMD) read the next wx's row during
+  // this one's transform; the extra registers spill to AGPRs
+  constexpr bool kPrefetch = E > 24;
+  float2 nxt[kPrefetch ? E : 1];
+  if constexpr (kPrefetch)
+    if (wave < nwx) load_row(nxt, wave);
+  for (int k = wave; k < nwx; k += RW) {
+    float2 v[E];
+    if constexpr (kPrefetch) {
+#pragma unroll
+      for (int j = 0; j < E; ++j) v[j] = nxt[j];
+      if (k + RW < nwx) load_row(nxt, k + RW);
+    } else {
+      load_row(v, k);
     }
     fft_fwd<P>(v, buf, ctx);
     const float2* hx = Hx + (size_t)k * N + lane;

@@ -67,6 +67,27 @@ def test_batch_with_explicit_tunings_and_oracle(oracle):
             assert rms < 1e-3
 
 
+def test_batch_long_transform_plans_are_bitwise_equal_to_single_runs():
+    # plans with more than 24 elements per lane (64 x 36 = 2304 points) take
+    # the kernels' read-ahead branches; forced here on small images so the
+    # batched and single-frame kernels run them on the same inputs
+    sizes = [(160, 120), (96, 200)]
+    T = 3
+    frames, phis = _inputs(sizes, T, seed0=700)
+    plans = {p["id"]: p for p in glsu.fft_plans()}
+    long_plan = [i for i, p in plans.items() if p["T"] == 64 and p["E"] > 24][0]
+    tuning = dict(planx=long_plan, plany=long_plan, cols_waves=2, rows_waves=2)
+    b = glsu.DPRABatch(phis, 1, tunings=[tuning, tuning])
+    outs = b(frames)
+    for i, (W, H) in enumerate(sizes):
+        assert b.tuning(i)["plany"] == long_plan and b.tuning(i)["planx"] == long_plan
+        ref = glsu.DPRA(phis[i], 1, tuning=b.tuning(i))(frames[i]).cpu().numpy()
+        got = outs[i].cpu().numpy()
+        say(f"[batch long plan {plans[long_plan]['N']}] seq {i} {W}x{H}: "
+            f"bitwise equal={np.array_equal(got, ref)}")
+        assert np.array_equal(got, ref), np.abs(got - ref).max()
+
+
 def test_batch_uneven_workload_shape():
     """Sizes drawn from the uneven set (scaled down to 3 sizes, 12 sequences,
     3 frames): runs, and three sequences equal their single-handle runs."""

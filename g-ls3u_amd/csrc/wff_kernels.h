@@ -154,19 +154,6 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
     acc[j] = make_float2(0.f, 0.f);
   }
   fft_fwd<P>(ah, buf, ctx);
-  // threshold on image rows; false when no slot of the wavefront kept a sample
-  auto threshold = [&](float2(&v)[E]) {
-    bool live = false;
-#pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const int y = lane + T * j;
-      const float m2 = v[j].x * v[j].x + v[j].y * v[j].y;
-      const bool keep = y < H && m2 >= thr2;
-      if (!keep) v[j] = make_float2(0.f, 0.f);
-      live |= keep;
-    }
-    return __builtin_amdgcn_ballot_w64(live) != 0;
-  };
   if constexpr (E > 24) {
     // long transforms run at one wavefront per SIMD, so nothing else hides an
     // L2 round trip: the wy spectrum is read once and held across both of its
@@ -180,7 +167,16 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
         v[j] = cmul(ah[j], h[j]);
       }
       fft_inv<P>(v, buf, ctx);
-      if (!threshold(v)) continue;
+      bool live = false;
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const int y = lane + T * j;
+        const float m2 = v[j].x * v[j].x + v[j].y * v[j].y;
+        const bool keep = y < H && m2 >= thr2;
+        if (!keep) v[j] = make_float2(0.f, 0.f);
+        live |= keep;
+      }
+      if (__builtin_amdgcn_ballot_w64(live) == 0) continue;
       fft_fwd<P>(v, buf, ctx);
 #pragma unroll
       for (int j = 0; j < E; ++j) {
@@ -195,10 +191,19 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
 #pragma unroll
       for (int j = 0; j < E; ++j) v[j] = cmul(ah[j], hy[j * T]);
       fft_inv<P>(v, buf, ctx);
+      bool live = false;
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const int y = lane + T * j;
+        const float m2 = v[j].x * v[j].x + v[j].y * v[j].y;
+        const bool keep = y < H && m2 >= thr2;
+        if (!keep) v[j] = make_float2(0.f, 0.f);
+        live |= keep;
+      }
       // a column whose thresholded signal is zero everywhere adds exactly
       // nothing: skip its transform when no slot of the wavefront has a
       // surviving sample (wave-uniform branch)
-      if (!threshold(v)) continue;
+      if (__builtin_amdgcn_ballot_w64(live) == 0) continue;
       fft_fwd<P>(v, buf, ctx);
       // re-read the window spectrum instead of keeping 2E VGPRs live across
       // the two transforms: the opaque copy of the lane offset defeats load CSE
@@ -207,9 +212,9 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
       const float2* hy2 = Hy + (size_t)q * N + l2;
 #pragma unroll
       for (int j = 0; j < E; ++j) {
-        const float2 hh = hy2[j * T];
-        acc[j].x += v[j].x * hh.x - v[j].y * hh.y;
-        acc[j].y += v[j].x * hh.y + v[j].y * hh.x;
+        const float2 h = hy2[j * T];
+        acc[j].x += v[j].x * h.x - v[j].y * h.y;
+        acc[j].y += v[j].x * h.y + v[j].y * h.x;
       }
     }
   }
@@ -269,36 +274,50 @@ __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ D,
   float2 acc[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) acc[j] = make_float2(0.f, 0.f);
-  auto load_row = [&](float2(&r)[E], int k) {
-    const float2* Dk = D + plane * k;
+  if constexpr (E > 24) {
+    // long transforms (one wavefront per SIMD) read the next wx's row during
+    // this one's transform; the extra registers spill to AGPRs
+    auto load_row = [&](float2(&r)[E], int k) {
+      const float2* Dk = D + plane * k;
 #pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const int x = lane + T * j;
-      r[j] = x < W ? Dk[tile_index(y, x, H)] : make_float2(0.f, 0.f);
-    }
-  };
-  // long transforms (one wavefront per SIMD) read the next wx's row during
-  // this one's transform; the extra registers spill to AGPRs
-  constexpr bool kPrefetch = E > 24;
-  float2 nxt[kPrefetch ? E : 1];
-  if constexpr (kPrefetch)
+      for (int j = 0; j < E; ++j) {
+        const int x = lane + T * j;
+        r[j] = x < W ? Dk[tile_index(y, x, H)] : make_float2(0.f, 0.f);
+      }
+    };
+    float2 nxt[E];
     if (wave < nwx) load_row(nxt, wave);
-  for (int k = wave; k < nwx; k += RW) {
-    float2 v[E];
-    if constexpr (kPrefetch) {
+    for (int k = wave; k < nwx; k += RW) {
+      float2 v[E];
 #pragma unroll
       for (int j = 0; j < E; ++j) v[j] = nxt[j];
       if (k + RW < nwx) load_row(nxt, k + RW);
-    } else {
-      load_row(v, k);
-    }
-    fft_fwd<P>(v, buf, ctx);
-    const float2* hx = Hx + (size_t)k * N + lane;
+      fft_fwd<P>(v, buf, ctx);
+      const float2* hx = Hx + (size_t)k * N + lane;
 #pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const float2 h = hx[j * T];
-      acc[j].x += v[j].x * h.x - v[j].y * h.y;
-      acc[j].y += v[j].x * h.y + v[j].y * h.x;
+      for (int j = 0; j < E; ++j) {
+        const float2 h = hx[j * T];
+        acc[j].x += v[j].x * h.x - v[j].y * h.y;
+        acc[j].y += v[j].x * h.y + v[j].y * h.x;
+      }
+    }
+  } else {
+    for (int k = wave; k < nwx; k += RW) {
+      const float2* Dk = D + plane * k;
+      float2 v[E];
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const int x = lane + T * j;
+        v[j] = x < W ? Dk[tile_index(y, x, H)] : make_float2(0.f, 0.f);
+      }
+      fft_fwd<P>(v, buf, ctx);
+      const float2* hx = Hx + (size_t)k * N + lane;
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const float2 h = hx[j * T];
+        acc[j].x += v[j].x * h.x - v[j].y * h.y;
+        acc[j].y += v[j].x * h.y + v[j].y * h.x;
+      }
     }
   }
   if (RW > 1) {

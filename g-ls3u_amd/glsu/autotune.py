@@ -11,9 +11,9 @@ The search is staged because the kernels are independent:
   1. columns: (plany, cols_waves), timed on the column kernel alone;
   2. rows:    (planx, rows_waves), timed on the forward + inverse row kernels.
 Timings come from glsu_dpra_profile (HIP events between kernels) over a few
-synthetic frames of the requested size, taken at the start of a synthetic
-sequence and after the chain has run through its first frames (the times are
-summed; "ms" in the result is that sum).  Results are cached per device, size
+synthetic frames of the requested size, taken at three points of a synthetic
+sequence with the chain's own state there (the times are summed; "ms" in the
+result is that sum).  Results are cached per device, size
 and WFF parameters in a JSON file.
 """
 import json
@@ -94,24 +94,30 @@ def _time(samples, params, tuning, reps):
     return used, total
 
 
-# Timing samples: the first frames of a synthetic sequence, and frames _LATE_T0
-# onwards with the reference phase the chain itself reached there.  The column
-# kernel skips transforms of columns whose thresholded signal is zero; how
-# many there are, and how they cluster, changes as the chain's reference phase
-# evolves, and the launch shape that wins on the first frames can lose on the
-# later ones (measured at 1024^2: 2 vs 4 column wavefronts swap order after
-# about 70 frames).
-_LATE_T0 = 96
+# Timing samples: frames at three points of a synthetic sequence (its start,
+# frame 100 and frame 180), each with the reference phase the chain itself
+# reached there.  The column kernel skips transforms of columns whose
+# thresholded signal is zero; how many there are, and how they cluster,
+# changes as the chain's reference phase evolves, and the launch shape that
+# wins on the first frames can lose on the later ones (measured at 1024^2: 2
+# vs 4 column wavefronts swap order after about 70 frames), so the samples
+# span a 200-frame sequence the way its average does.
+_SAMPLE_T0 = (0, 100, 180)
 
 
 def _samples(width, height, frames, params):
     phi0 = synth.phase(width, height, 3.0, 0.05)
-    seq = gen_frames(width, height, _LATE_T0 + frames, seed=1234)
-    d = DPRA(phi0, 1, params=params)
-    d(seq[:_LATE_T0])  # advance the chain with the default launch shapes
-    phr, _ = d.get_state()
+    last = _SAMPLE_T0[-1]
+    seq = gen_frames(width, height, last + frames, seed=1234)
+    out, t, d = [], 0, DPRA(phi0, 1, params=params)  # default launch shapes
+    for t0 in _SAMPLE_T0:
+        if t0 > t:
+            d(seq[t:t0])  # advance the chain to frame t0
+            t = t0
+        phr = phi0 if t0 == 0 else d.get_state()[0].astype(np.float64)
+        out.append((phr, seq[t0:t0 + frames]))
     d.close()
-    return [(phi0, seq[:frames]), (phr.astype(np.float64), seq[_LATE_T0:])]
+    return out
 
 
 def tune(width, height, params=None, frames=4, reps=3, cache=None, force=False, log=None):

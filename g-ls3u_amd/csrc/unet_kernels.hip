@@ -58,15 +58,18 @@ __device__ __forceinline__ bf16x8 load8_clamped(const __bf16* __restrict__ X1, i
 }
 
 // ------------------------------------------------------------------------
-// 3x3 convolution, implicit GEMM: M = pixels, N = output channels, K = 9*Cin.
+// 3x3 convolution, implicit GEMM over pixels x output channels, K = 9*Cin.
 // Workgroup: 4 waves, output tile 4 rows x 64 pixels x NCO channels; wave w
 // owns row w.  Per 32-channel chunk the input tile (6 x 66 pixels x 32
 // channels, 25 KB) is staged in LDS; each 16-byte group of 8 channels of LDS
 // pixel p sits at group slot g ^ ((p >> 1) & 3), which keeps the MFMA operand
 // reads (16 consecutive pixels, one group) free of bank conflicts without
-// padding.  The weight operands are read straight from global memory (they
-// are small and shared by every workgroup, so they stay in L1/L2), one tap
-// ahead of their use.  Each wave issues 4 x (NCO/16) MFMAs per tap.
+// padding.  Weights: 32-channel tiles read them straight from global memory
+// (small and shared by every workgroup, so they stay in L1/L2), one tap ahead
+// of their use; 64-channel tiles stage them in LDS one row of three taps at a
+// time.  The weights are the MFMA A operand and the pixels the B operand, so
+// a lane's accumulators hold four consecutive output channels of one pixel.
+// Each wave issues 4 x (NCO/16) MFMAs per tap.
 // MODE 0: y = relu(acc + bias).  MODE 1: y = acc * (mask > 0) (mask may be
 // null: y = acc).  Used for the forward pass (weights [Cout][9][Cin]) and
 // for the input gradient (dY as input, flipped transposed weights).

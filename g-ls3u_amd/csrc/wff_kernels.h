@@ -456,9 +456,10 @@ __device__ __forceinline__ int swap_parity(int t, int rr) {
 }
 
 // (two resident workgroups per CU keep the short plans at 2 waves/SIMD; the
-// long plans need the whole register file of one wave, or they spill)
+// plans of 24 points per lane and more need the whole register file of one
+// wave, or they spill)
 template <class P>
-__global__ __launch_bounds__(64 * kRowWavesMax, P::E > 24 ? 1 : 2) void wff_rows_inv_batch_kernel(
+__global__ __launch_bounds__(64 * kRowWavesMax, P::E >= 24 ? 1 : 2) void wff_rows_inv_batch_kernel(
     const BatchSeg* __restrict__ tab, int nseg, int nrows, const float2* __restrict__ Hx,
     const float2* __restrict__ tw_g, int nwx, float scale, const int* __restrict__ frame) {
   extern __shared__ float2 smem[];

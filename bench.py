@@ -269,9 +269,9 @@ def bench_wfr(args, glsu, torch, np):
     nwx, nwy, Ny = info["nwx"], info["nwy"], info["ny"]
     stage_ms = [w.time_kernel(f, k, reps=3) for k in (1, 2, 3)]
     col_ms = stage_ms[1]
-    # per (wx, column): 1 + nwy FFTs of length Ny, and per pair the spectrum
-    # product (6 flop/point) and the magnitude compare (4)
-    flops = nwx * W * ((1 + nwy) * 5.0 * Ny * math.log2(Ny) + nwy * Ny * 10.0)
+    # per (wx, column): 1 + nwy FFTs of length Ny, and per pair the product
+    # with the real window spectrum (2 flop/point) and the magnitude compare (4)
+    flops = nwx * W * ((1 + nwy) * 5.0 * Ny * math.log2(Ny) + nwy * Ny * 6.0)
     achieved = flops / (col_ms * 1e-3) / 1e12
     line = {
         "metric": "WFR2 fields/s @1024^2 (161 x 161 pairs)", "value": round(fps, 3),
@@ -400,9 +400,9 @@ def bench_batch(args, glsu, torch, np):
 def col_kernel_flops(W, H, nwx, nwy, Ny):
     """Algorithmic FLOP of one launch of the column kernel: per (wx, column)
     item, 2 + 2*nwy complex FFTs of length Ny (5 N log2 N each) plus per pair
-    two spectrum multiplies (6 flop each), the threshold test (3) and the
-    accumulate (2) per point (16 flop/point/pair in total)."""
-    per_item = (2 + 2 * nwy) * 5.0 * Ny * math.log2(Ny) + nwy * Ny * 16.0
+    two products with the real window spectrum (2 flop each), the threshold
+    test (3) and the accumulate (2) per point (9 flop/point/pair in total)."""
+    per_item = (2 + 2 * nwy) * 5.0 * Ny * math.log2(Ny) + nwy * Ny * 9.0
     return nwx * W * per_item
 
 

@@ -38,6 +38,7 @@ __host__ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * c
   return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 __host__ __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ float2 rmul(float2 a, float h) { return make_float2(a.x * h, a.y * h); }
 __host__ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
 // ------------------------------------------------- compile-time trigonometry

@@ -31,9 +31,10 @@ struct WffPlan {
   int Nx = 0, Ny = 0;
   int cols_waves = 4, rows_waves = 1;  // launch shapes
   // device buffers
-  // window spectra / N, lane-major: [w][j][lane] = bin kbase(lane) + koff(j)
-  float2* d_Hx = nullptr;   // [nwx][Nx]
-  float2* d_Hy = nullptr;   // [nwy][Ny]
+  // window spectra / N, lane-major: [w][j][lane] = bin kbase(lane) + koff(j).
+  // Real: the window is even, so its modulated spectrum has no imaginary part.
+  float* d_Hx = nullptr;    // [nwx][Nx]
+  float* d_Hy = nullptr;    // [nwy][Ny]
   float2* d_twx = nullptr;  // W_Nx^k
   float2* d_twy = nullptr;  // W_Ny^k
   float2* d_A = nullptr;    // [nwx] tiled planes: row-convolved field
@@ -181,7 +182,8 @@ struct BatchClass {
   int planx, plany, cols_waves, rows_waves;
   int nwx, nwy;
   float scale;
-  const float2 *Hx, *Hy, *twx, *twy;
+  const float *Hx, *Hy;
+  const float2 *twx, *twy;
   int grid_rows, grid_cols;  // persistent grid sizes (multiples of 8)
 };
 hipError_t launch_rows_fwd_batch(const BatchClass& c, hipStream_t s);

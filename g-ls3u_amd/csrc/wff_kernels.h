@@ -53,7 +53,7 @@ __device__ __forceinline__ size_t tile_index(int y, int x, int H) {
 template <class P>
 __device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ frow,
                                               float2* __restrict__ A,
-                                              const float2* __restrict__ Hx,
+                                              const float* __restrict__ Hx,
                                               const float2* __restrict__ tw_g, int W, int H,
                                               int nwx, size_t plane, int y, int lane,
                                               float2* smem) {
@@ -85,9 +85,9 @@ __device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ frow,
   // before this wx's plane stores (the wait for a load also waits for every
   // store issued before it); the extra registers spill to AGPRs
   constexpr bool kPrefetch = E > 24;
-  float2 h[E];  // used only when kPrefetch
+  float h[E];  // used only when kPrefetch
   auto load_h = [&](int k) {
-    const float2* hx = Hx + (size_t)k * N + lane;  // lane-major: one 512-byte row per j
+    const float* hx = Hx + (size_t)k * N + lane;  // lane-major: one 512-byte row per j
 #pragma unroll
     for (int j = 0; j < E; ++j) h[j] = hx[j * T];
   };
@@ -97,11 +97,11 @@ __device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ frow,
     float2 v[E];
     if constexpr (kPrefetch) {
 #pragma unroll
-      for (int j = 0; j < E; ++j) v[j] = cmul(z[j], h[j]);
+      for (int j = 0; j < E; ++j) v[j] = rmul(z[j], h[j]);
     } else {
-      const float2* hx = Hx + (size_t)k * N + lane;
+      const float* hx = Hx + (size_t)k * N + lane;
 #pragma unroll
-      for (int j = 0; j < E; ++j) v[j] = cmul(z[j], hx[j * T]);
+      for (int j = 0; j < E; ++j) v[j] = rmul(z[j], hx[j * T]);
     }
     fft_inv<P>(v, buf, ctx);
     if constexpr (kPrefetch)
@@ -117,7 +117,7 @@ __device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ frow,
 
 template <class P>
 __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_kernel(
-    const float2* __restrict__ f, float2* __restrict__ A, const float2* __restrict__ Hx,
+    const float2* __restrict__ f, float2* __restrict__ A, const float* __restrict__ Hx,
     const float2* __restrict__ tw_g, int W, int H, int nwx, size_t plane,
     const int* __restrict__ skip) {
   extern __shared__ float2 smem[];
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_kernel(
 // caller before the first item.
 template <class P>
 __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* __restrict__ D,
-                                          const float2* __restrict__ Hy,
+                                          const float* __restrict__ Hy,
                                           const float2* __restrict__ tw_g, float thr, int W,
                                           int H, int nwx, int nwy, size_t plane, int item,
                                           int lane, const float2* ttab, float2* buf) {
@@ -159,12 +159,13 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
     // L2 round trip: the wy spectrum is read once and held across both of its
     // products (the extra registers spill to AGPRs, cheaper than the stall)
     for (int q = 0; q < nwy; ++q) {
-      const float2* hy = Hy + (size_t)q * N + lane;
-      float2 h[E], v[E];
+      const float* hy = Hy + (size_t)q * N + lane;
+      float h[E];
+      float2 v[E];
 #pragma unroll
       for (int j = 0; j < E; ++j) {
         h[j] = hy[j * T];
-        v[j] = cmul(ah[j], h[j]);
+        v[j] = rmul(ah[j], h[j]);
       }
       fft_inv<P>(v, buf, ctx);
       bool live = false;
@@ -180,16 +181,16 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
       fft_fwd<P>(v, buf, ctx);
 #pragma unroll
       for (int j = 0; j < E; ++j) {
-        acc[j].x += v[j].x * h[j].x - v[j].y * h[j].y;
-        acc[j].y += v[j].x * h[j].y + v[j].y * h[j].x;
+        acc[j].x += v[j].x * h[j];
+        acc[j].y += v[j].y * h[j];
       }
     }
   } else {
     for (int q = 0; q < nwy; ++q) {
-      const float2* hy = Hy + (size_t)q * N + lane;
+      const float* hy = Hy + (size_t)q * N + lane;
       float2 v[E];
 #pragma unroll
-      for (int j = 0; j < E; ++j) v[j] = cmul(ah[j], hy[j * T]);
+      for (int j = 0; j < E; ++j) v[j] = rmul(ah[j], hy[j * T]);
       fft_inv<P>(v, buf, ctx);
       bool live = false;
 #pragma unroll
@@ -205,16 +206,16 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
       // surviving sample (wave-uniform branch)
       if (__builtin_amdgcn_ballot_w64(live) == 0) continue;
       fft_fwd<P>(v, buf, ctx);
-      // re-read the window spectrum instead of keeping 2E VGPRs live across
+      // re-read the window spectrum instead of keeping E VGPRs live across
       // the two transforms: the opaque copy of the lane offset defeats load CSE
       int l2 = lane;
       __asm__ volatile("" : "+v"(l2));
-      const float2* hy2 = Hy + (size_t)q * N + l2;
+      const float* hy2 = Hy + (size_t)q * N + l2;
 #pragma unroll
       for (int j = 0; j < E; ++j) {
-        const float2 h = hy2[j * T];
-        acc[j].x += v[j].x * h.x - v[j].y * h.y;
-        acc[j].y += v[j].x * h.y + v[j].y * h.x;
+        const float h = hy2[j * T];
+        acc[j].x += v[j].x * h;
+        acc[j].y += v[j].y * h;
       }
     }
   }
@@ -229,7 +230,7 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
 
 template <class P, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
-    const float2* __restrict__ A, float2* __restrict__ D, const float2* __restrict__ Hy,
+    const float2* __restrict__ A, float2* __restrict__ D, const float* __restrict__ Hy,
     const float2* __restrict__ tw_g, const float* __restrict__ thr_p, int W, int H, int nwx,
     int nwy, size_t plane, const int* __restrict__ skip) {
   constexpr int T = P::T, E = P::E;
@@ -259,7 +260,7 @@ __global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
 //   t = angle(F); phicurr = wrap(phiref + t); dphi = dphiref + t (two-float sum).
 template <class P, int MODE>
 __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ D,
-                                              const float2* __restrict__ Hx,
+                                              const float* __restrict__ Hx,
                                               const float2* __restrict__ tw_g, int W, int H,
                                               int nwx, size_t plane, float scale,
                                               float2* __restrict__ filtered, const UpdateArgs& u,
@@ -293,12 +294,12 @@ __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ D,
       for (int j = 0; j < E; ++j) v[j] = nxt[j];
       if (k + RW < nwx) load_row(nxt, k + RW);
       fft_fwd<P>(v, buf, ctx);
-      const float2* hx = Hx + (size_t)k * N + lane;
+      const float* hx = Hx + (size_t)k * N + lane;
 #pragma unroll
       for (int j = 0; j < E; ++j) {
-        const float2 h = hx[j * T];
-        acc[j].x += v[j].x * h.x - v[j].y * h.y;
-        acc[j].y += v[j].x * h.y + v[j].y * h.x;
+        const float h = hx[j * T];
+        acc[j].x += v[j].x * h;
+        acc[j].y += v[j].y * h;
       }
     }
   } else {
@@ -311,12 +312,12 @@ __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ D,
         v[j] = x < W ? Dk[tile_index(y, x, H)] : make_float2(0.f, 0.f);
       }
       fft_fwd<P>(v, buf, ctx);
-      const float2* hx = Hx + (size_t)k * N + lane;
+      const float* hx = Hx + (size_t)k * N + lane;
 #pragma unroll
       for (int j = 0; j < E; ++j) {
-        const float2 h = hx[j * T];
-        acc[j].x += v[j].x * h.x - v[j].y * h.y;
-        acc[j].y += v[j].x * h.y + v[j].y * h.x;
+        const float h = hx[j * T];
+        acc[j].x += v[j].x * h;
+        acc[j].y += v[j].y * h;
       }
     }
   }
@@ -370,7 +371,7 @@ __device__ __forceinline__ void rows_inv_body(const float2* __restrict__ D,
 
 template <class P, int MODE>
 __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_inv_kernel(
-    const float2* __restrict__ D, const float2* __restrict__ Hx, const float2* __restrict__ tw_g,
+    const float2* __restrict__ D, const float* __restrict__ Hx, const float2* __restrict__ tw_g,
     int W, int H, int nwx, size_t plane, float scale, float2* __restrict__ filtered,
     UpdateArgs u, const int* __restrict__ skip) {
   extern __shared__ float2 smem[];
@@ -414,7 +415,7 @@ __device__ __forceinline__ int seg_find(const BatchSeg* __restrict__ tab, int n,
 
 template <class P>
 __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_batch_kernel(
-    const BatchSeg* __restrict__ tab, int nseg, int nrows, const float2* __restrict__ Hx,
+    const BatchSeg* __restrict__ tab, int nseg, int nrows, const float* __restrict__ Hx,
     const float2* __restrict__ tw_g, int nwx) {
   extern __shared__ float2 smem[];
   for (int v = blockIdx.x; v < nrows; v += gridDim.x) {
@@ -428,7 +429,7 @@ __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_batch_kernel(
 
 template <class P, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void wff_cols_batch_kernel(
-    const BatchSeg* __restrict__ tab, int nseg, int nblk, const float2* __restrict__ Hy,
+    const BatchSeg* __restrict__ tab, int nseg, int nblk, const float* __restrict__ Hy,
     const float2* __restrict__ tw_g, int nwx, int nwy) {
   constexpr int T = P::T, E = P::E;
   extern __shared__ float2 smem[];
@@ -460,7 +461,7 @@ __device__ __forceinline__ int swap_parity(int t, int rr) {
 // wave, or they spill)
 template <class P>
 __global__ __launch_bounds__(64 * kRowWavesMax, P::E >= 24 ? 1 : 2) void wff_rows_inv_batch_kernel(
-    const BatchSeg* __restrict__ tab, int nseg, int nrows, const float2* __restrict__ Hx,
+    const BatchSeg* __restrict__ tab, int nseg, int nrows, const float* __restrict__ Hx,
     const float2* __restrict__ tw_g, int nwx, float scale, const int* __restrict__ frame) {
   extern __shared__ float2 smem[];
   const int t = *frame;

@@ -127,8 +127,8 @@ static const FftPlanInfo* find_plan(int id) {
 // spectral bin kbase(lane) + koff(j) of row r, so that register j of every
 // lane of a transform is one contiguous T-element run (a coalesced load).
 template <class P>
-static void to_lane_major(std::vector<float2>& h, int rows) {
-  std::vector<float2> out(h.size());
+static void to_lane_major(std::vector<float>& h, int rows) {
+  std::vector<float> out(h.size());
   for (int r = 0; r < rows; ++r)
     for (int j = 0; j < P::E; ++j)
       for (int l = 0; l < P::T; ++l)
@@ -136,7 +136,7 @@ static void to_lane_major(std::vector<float2>& h, int rows) {
   h.swap(out);
 }
 
-static bool lane_major(int plan, std::vector<float2>& h, int rows) {
+static bool lane_major(int plan, std::vector<float>& h, int rows) {
   switch (plan) {
 #define BODY                \
   to_lane_major<P>(h, rows); \
@@ -151,19 +151,19 @@ static bool lane_major(int plan, std::vector<float2>& h, int rows) {
 
 // (1/N) * DFT of the centred, modulated 1-D window g1(d) e^{j w d}, d in [-s, s]:
 // H[k] = (1/N) sum_d g1(d) e^{j w d} e^{-2 pi i k d / N}
-static std::vector<float2> window_spectrum(const std::vector<double>& g1, int s, double w, int N) {
-  std::vector<float2> out(N);
+//      = (1/N) sum_d g1(d) cos((w - 2 pi k / N) d)
+// g1 is even, so the sine terms of d and -d cancel: the spectrum is real.
+static std::vector<float> window_spectrum(const std::vector<double>& g1, int s, double w, int N) {
+  std::vector<float> out(N);
   for (int k = 0; k < N; ++k) {
-    double re = 0, im = 0;
+    double re = 0;
     for (int d = -s; d <= s; ++d) {
       long kd = ((long)k * d) % N;
       if (kd < 0) kd += N;
       const double a = w * d - 2.0 * M_PI * (double)kd / (double)N;
-      const double g = g1[d + s];
-      re += g * std::cos(a);
-      im += g * std::sin(a);
+      re += g1[d + s] * std::cos(a);
     }
-    out[k] = make_float2((float)(re / N), (float)(im / N));
+    out[k] = (float)(re / N);
   }
   return out;
 }
@@ -249,7 +249,7 @@ int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh,
   p.plane_elems = (size_t)wb * kTileCols * H;
   // spectra tables
   auto gx = norm_window(sigx, p.sx), gy = norm_window(sigy, p.sy);
-  std::vector<float2> hx, hy;
+  std::vector<float> hx, hy;
   for (int k = 0; k < p.nwx; ++k) {
     auto t = window_spectrum(gx, p.sx, wxl + (double)k * wxi, p.Nx);
     hx.insert(hx.end(), t.begin(), t.end());
@@ -290,7 +290,8 @@ int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh,
 void wff_plan_free(WffPlan& p) {
   hipFree(p.d_Hx); hipFree(p.d_Hy); hipFree(p.d_twx); hipFree(p.d_twy);
   hipFree(p.d_A); hipFree(p.d_D); hipFree(p.d_thr); hipFree(p.d_red);
-  p.d_Hx = p.d_Hy = p.d_twx = p.d_twy = p.d_A = p.d_D = nullptr;
+  p.d_Hx = p.d_Hy = nullptr;
+  p.d_twx = p.d_twy = p.d_A = p.d_D = nullptr;
   p.d_thr = p.d_red = nullptr;
 }
 

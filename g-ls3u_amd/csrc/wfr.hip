@@ -30,7 +30,7 @@ constexpr uint16_t kNoPair = 0xFFFF;
 template <class P>
 __device__ __forceinline__ void cols_wfr_body(const float2* __restrict__ A,
                                               float2* __restrict__ D, uint16_t* __restrict__ I,
-                                              const float2* __restrict__ Hy,
+                                              const float* __restrict__ Hy,
                                               const float2* __restrict__ tw_g, int H, int nwy,
                                               size_t plane, int k, int x, int lane,
                                               const float2* ttab, float2* buf) {
@@ -50,10 +50,10 @@ __device__ __forceinline__ void cols_wfr_body(const float2* __restrict__ A,
   }
   fft_fwd<P>(ah, buf, ctx);
   for (int q = 0; q < nwy; ++q) {
-    const float2* hy = Hy + (size_t)q * N + lane;  // lane-major window spectra
+    const float* hy = Hy + (size_t)q * N + lane;  // lane-major window spectra
     float2 v[E];
 #pragma unroll
-    for (int j = 0; j < E; ++j) v[j] = cmul(ah[j], hy[j * T]);
+    for (int j = 0; j < E; ++j) v[j] = rmul(ah[j], hy[j * T]);
     fft_inv<P>(v, buf, ctx);
 #pragma unroll
     for (int j = 0; j < E; ++j) {
@@ -78,7 +78,7 @@ __device__ __forceinline__ void cols_wfr_body(const float2* __restrict__ A,
 template <class P, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void wfr_cols_kernel(
     const float2* __restrict__ A, float2* __restrict__ D, uint16_t* __restrict__ I,
-    const float2* __restrict__ Hy, const float2* __restrict__ tw_g, int W, int H, int nwx,
+    const float* __restrict__ Hy, const float2* __restrict__ tw_g, int W, int H, int nwx,
     int nwy, size_t plane) {
   constexpr int T = P::T, E = P::E;
   extern __shared__ float2 smem[];

@@ -2,7 +2,8 @@
 threshold :977-1022): brute-force direct convolution, closed forms and
 invariants.  Also pins the exactness claims the GPU design rests on
 (centred circular convolution of any length N >= size + s equals the linear
-one; separable 1-D spectra equal FFT2 of the 2-D window)."""
+one; separable 1-D spectra equal FFT2 of the 2-D window; the centred window's
+spectra are real)."""
 import numpy as np
 import pytest
 
@@ -91,9 +92,21 @@ def test_wff2_separable_equals_literal_fft2_gwave(oracle):
     assert np.abs(a - b).max() < 1e-13 * np.abs(a).max()
 
 
-def _centred_separable_wff2(f, sig, lo, hi, st, Nx, Ny, thr):
+def _centred_spectrum(sig, w, N, shift=0):
+    """(unscaled) DFT of the centred, modulated window g(d) e^{jwd}, d in
+    [-s, s], placed at d + shift (mod N)."""
+    s = bf.c_round(3 * sig)
+    d = np.arange(-s, s + 1)
+    g = np.exp(-(d * d) / 2 / sig / sig)
+    g /= np.sqrt((g * g).sum())
+    k = np.arange(N)[:, None]
+    return (g[None, :] * np.exp(1j * (w * d[None, :] - 2 * np.pi * ((k * (d[None, :] + shift)) % N) / N))).sum(1)
+
+
+def _centred_separable_wff2(f, sig, lo, hi, st, Nx, Ny, thr, real=False):
     """The GPU formulation in numpy: centred circular convolutions of lengths
-    Nx (rows) and Ny (columns), pair loop regrouped as rows->columns->rows."""
+    Nx (rows) and Ny (columns), pair loop regrouped as rows->columns->rows.
+    real: keep only the real part of the window spectra, as the GPU does."""
     H, W = f.shape
     s = bf.c_round(3 * sig)
     n = int((hi - lo) * (1 / st)) + 1
@@ -108,6 +121,9 @@ def _centred_separable_wff2(f, sig, lo, hi, st, Nx, Ny, thr):
     ws = [lo + i * st for i in range(n)]
     Hx = [spec(w, Nx) for w in ws]
     Hy = [spec(w, Ny) for w in ws]
+    if real:
+        Hx = [h.real.astype(complex) for h in Hx]
+        Hy = [h.real.astype(complex) for h in Hy]
     fp = np.zeros((H, Nx), complex)
     fp[:, :W] = f
     Z = np.fft.fft(fp, axis=1)
@@ -154,3 +170,24 @@ def test_padding_error(oracle):
     with pytest.raises(ValueError):
         oracle.WFF2(4096 - 60 + 2, 64, 10.0)
     oracle.WFF2(4096 - 60, 16, 10.0, -0.1, 0.1, 0.1)  # boundary: fits
+
+
+@pytest.mark.parametrize("sig,w,N", [(10.0, -0.2, 1280), (10.0, 0.1, 1280), (12.0, 0.45, 2304),
+                                     (10.0, -2.0, 1280), (2.0, 0.3, 46)])
+def test_centred_window_spectrum_is_real(sig, w, N):
+    # the window is even, so the sine terms of d and -d cancel for every bin
+    h = _centred_spectrum(sig, w, N)
+    assert np.abs(h.imag).max() < 1e-13 * np.abs(h).max()
+    # a window placed one sample off centre carries a linear phase: not real
+    h1 = _centred_spectrum(sig, w, N, shift=1)
+    assert np.abs(h1.imag).max() > 1e-3 * np.abs(h1).max()
+
+
+@pytest.mark.parametrize("Nx,Ny", [(46, 38), (97, 53)])
+def test_real_window_spectra_reproduce_the_oracle(oracle, Nx, Ny):
+    # the GPU stores only the real part of the spectra; the result is unchanged
+    H, W, sig = 32, 40, 2.0
+    f = _field(H, W, 9)
+    ref = oracle.WFF2(W, H, sig, -0.6, 0.6, 0.3)(f)
+    out = _centred_separable_wff2(f, sig, -0.6, 0.6, 0.3, Nx, Ny, 2 * np.sqrt(3), real=True)
+    assert np.abs(out - ref).max() < 1e-12

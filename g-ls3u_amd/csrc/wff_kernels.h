@@ -87,7 +87,7 @@ __device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ frow,
   constexpr bool kPrefetch = E > 24;
   float h[E];  // used only when kPrefetch
   auto load_h = [&](int k) {
-    const float* hx = Hx + (size_t)k * N + lane;  // lane-major: one 512-byte row per j
+    const float* hx = Hx + (size_t)k * N + lane;  // lane-major: one 256-byte row per j
 #pragma unroll
     for (int j = 0; j < E; ++j) h[j] = hx[j * T];
   };
@@ -132,8 +132,9 @@ __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_kernel(
 // One transform slot per (wx, column).  Ahat = FFT_y(A col); for each wy:
 // S = IFFT_y(Ahat*Hy); T = threshold(S) on image rows; acc += FFT_y(T)*Hy.
 // D col = IFFT_y(acc).  `ttab` (transition twiddles in LDS) is filled by the
-// caller before the first item.
-template <class P>
+// caller before the first item.  HOLD: keep each wy spectrum in registers
+// across both of its products, else read it again for the second one.
+template <class P, bool HOLD = true>
 __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* __restrict__ D,
                                           const float* __restrict__ Hy,
                                           const float2* __restrict__ tw_g, float thr, int W,
@@ -154,69 +155,51 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
     acc[j] = make_float2(0.f, 0.f);
   }
   fft_fwd<P>(ah, buf, ctx);
-  if constexpr (E > 24) {
-    // long transforms run at one wavefront per SIMD, so nothing else hides an
-    // L2 round trip: the wy spectrum is read once and held across both of its
-    // products (the extra registers spill to AGPRs, cheaper than the stall)
-    for (int q = 0; q < nwy; ++q) {
-      const float* hy = Hy + (size_t)q * N + lane;
-      float h[E];
-      float2 v[E];
+  // the wy spectrum (real: E registers) is read once and held across both of
+  // its products; at one wavefront per SIMD (E > 24) nothing else would hide
+  // a second L2 round trip, and at two the held copy still fits under 256 VGPRs
+  // in the single-image kernel (the persistent batched kernel's own state
+  // leaves no room for it there: it reads the spectrum again)
+  for (int q = 0; q < nwy; ++q) {
+    const float* hy = Hy + (size_t)q * N + lane;  // lane-major: one T-float run per j
+    float h[E];
+    float2 v[E];
 #pragma unroll
-      for (int j = 0; j < E; ++j) {
+    for (int j = 0; j < E; ++j) {
+      if constexpr (HOLD) {
         h[j] = hy[j * T];
         v[j] = rmul(ah[j], h[j]);
-      }
-      fft_inv<P>(v, buf, ctx);
-      bool live = false;
-#pragma unroll
-      for (int j = 0; j < E; ++j) {
-        const int y = lane + T * j;
-        const float m2 = v[j].x * v[j].x + v[j].y * v[j].y;
-        const bool keep = y < H && m2 >= thr2;
-        if (!keep) v[j] = make_float2(0.f, 0.f);
-        live |= keep;
-      }
-      if (__builtin_amdgcn_ballot_w64(live) == 0) continue;
-      fft_fwd<P>(v, buf, ctx);
-#pragma unroll
-      for (int j = 0; j < E; ++j) {
-        acc[j].x += v[j].x * h[j];
-        acc[j].y += v[j].y * h[j];
+      } else {
+        v[j] = rmul(ah[j], hy[j * T]);
       }
     }
-  } else {
-    for (int q = 0; q < nwy; ++q) {
-      const float* hy = Hy + (size_t)q * N + lane;
-      float2 v[E];
+    fft_inv<P>(v, buf, ctx);
+    bool live = false;
 #pragma unroll
-      for (int j = 0; j < E; ++j) v[j] = rmul(ah[j], hy[j * T]);
-      fft_inv<P>(v, buf, ctx);
-      bool live = false;
-#pragma unroll
-      for (int j = 0; j < E; ++j) {
-        const int y = lane + T * j;
-        const float m2 = v[j].x * v[j].x + v[j].y * v[j].y;
-        const bool keep = y < H && m2 >= thr2;
-        if (!keep) v[j] = make_float2(0.f, 0.f);
-        live |= keep;
-      }
-      // a column whose thresholded signal is zero everywhere adds exactly
-      // nothing: skip its transform when no slot of the wavefront has a
-      // surviving sample (wave-uniform branch)
-      if (__builtin_amdgcn_ballot_w64(live) == 0) continue;
-      fft_fwd<P>(v, buf, ctx);
-      // re-read the window spectrum instead of keeping E VGPRs live across
-      // the two transforms: the opaque copy of the lane offset defeats load CSE
+    for (int j = 0; j < E; ++j) {
+      const int y = lane + T * j;
+      const float m2 = v[j].x * v[j].x + v[j].y * v[j].y;
+      const bool keep = y < H && m2 >= thr2;
+      if (!keep) v[j] = make_float2(0.f, 0.f);
+      live |= keep;
+    }
+    // a column whose thresholded signal is zero everywhere adds exactly
+    // nothing: skip its transform when no slot of the wavefront has a
+    // surviving sample (wave-uniform branch)
+    if (__builtin_amdgcn_ballot_w64(live) == 0) continue;
+    fft_fwd<P>(v, buf, ctx);
+    if constexpr (!HOLD) {
+      // the opaque copy of the lane offset defeats load CSE with the first read
       int l2 = lane;
       __asm__ volatile("" : "+v"(l2));
       const float* hy2 = Hy + (size_t)q * N + l2;
 #pragma unroll
-      for (int j = 0; j < E; ++j) {
-        const float h = hy2[j * T];
-        acc[j].x += v[j].x * h;
-        acc[j].y += v[j].y * h;
-      }
+      for (int j = 0; j < E; ++j) h[j] = hy2[j * T];
+    }
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      acc[j].x += v[j].x * h[j];
+      acc[j].y += v[j].y * h[j];
     }
   }
   fft_inv<P>(acc, buf, ctx);
@@ -444,7 +427,7 @@ __global__ __launch_bounds__(64 * WAVES) void wff_cols_batch_kernel(
     const int lb = b - s.col0;  // plane lb % nwx, columns [(lb / nwx) * ipb, +ipb)
     const int x = (lb / nwx) * (blockDim.x / T) + slot;
     if (x < s.W)
-      cols_body<P>(s.A, s.D, Hy, tw_g, *s.thr, s.W, s.H, nwx, nwy, s.plane, (lb % nwx) * s.W + x,
+      cols_body<P, (P::E > 24)>(s.A, s.D, Hy, tw_g, *s.thr, s.W, s.H, nwx, nwy, s.plane, (lb % nwx) * s.W + x,
                    opaque_lane<P>(), ttab, buf);
   }
 }

@@ -51,35 +51,64 @@ WORKLOADS = {
 BATCH_SIZES = [256, 512, 768, 1024, 1536, 2048]
 
 
-def bench_learned(args, glsu, torch, np):
-    """Learned filter vs plain WFF2 on one 1024^2 sequence (500 frames, noise
-    std drawn from U[5, 25], seeds disjoint from the training pairs').  value =
-    frames/s of the learned chain (U-Net in place of WFF2, with the 64x64 WFF2
-    spot check and fallback every frame); also the plain chain's frames/s on
-    the same frames, and on sampled frames the wrapped-phase RMS of the
-    network's filtered phase, and of the unfiltered DPRA phase, against the
-    FP64 WFF2 of the same raw field (the CPU oracle)."""
-    size, T, wff, _ = WORKLOADS["learned1024"]
-    W = H = args.size or size
-    T = args.frames or T
+def host_info():
+    """Host cores usable by this process, the thread count the CPU legs use,
+    and the CPU model string."""
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        nproc = os.cpu_count() or 1
+    try:
+        share = int(os.environ.get("OMP_NUM_THREADS", "16"))
+    except ValueError:
+        share = 16
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": nproc, "os_cpu_count": os.cpu_count(), "threads": max(1, min(nproc, share)),
+            "cpu_model": model}
+
+
+LEARNED_TOL = 0.03
+
+
+def _learned_tile(W, H, t):
+    """Spot-check tile origin (x, y) of frame t: moves every frame."""
+    return ((t * 389) % (W - 64), (t * 613) % (H - 64))
+
+
+def learned_leg(args, glsu, torch, np, W=1024, T=500, with_accuracy=True, nthreads=1):
+    """The learned WFF2 surrogate on one 1024^2 sequence (500 frames, noise
+    std drawn from U[5, 25], seeds disjoint from the training pairs') with the
+    committed weights: frames/s of the learned chain (DPRA LS -> U-Net ->
+    phase update, with the 64x64 WFF2 spot check and fallback every frame) and
+    of the plain chain on the same frames.  Accuracy of the timed chain: the
+    chain is re-run (bitwise equal to the timed run: checked), and on sampled
+    frames its per-frame filtered phase -- the increment of its cumulative
+    output -- is compared with the FP64 WFF2 of the same raw field (the CPU
+    oracle), as is the unfiltered raw phase.  Returns a dict, or None when
+    the weights are missing."""
+    _, _, wff, _ = WORKLOADS["learned1024"]
+    H = W
+    path = glsu.learned.WEIGHTS
+    if not os.path.exists(path):
+        return None
     rng = np.random.default_rng(31337)
     amp0, noise = float(rng.uniform(2, 5)), float(rng.uniform(5, 25))
     params = glsu.wff_params(*wff)
     frames = glsu.gen_frames(W, H, T, 424242, amp0=amp0, noise=noise)
     phi0 = glsu.synth.phase(W, H, amp0, 0.05)
-    path = glsu.learned.WEIGHTS
-    trained_here = not os.path.exists(path)
-    if trained_here:  # no committed weights: a short training run (untimed)
-        x, u = glsu.learned.make_pairs(8, 25, 1024)
-        net = glsu.UNet(4, 32, seed=0)
-        glsu.learned.train(net, x, u, crops=8000)
-        del x, u
-    else:
-        net = glsu.UNet.load(path)
-    tol = 0.03
-    # plain chain, graph mode
-    d = glsu.DPRA(phi0, params=params)
+    net = glsu.UNet.load(path)
+    tol = LEARNED_TOL
     out = torch.empty((T, H, W), dtype=torch.float32, device="cuda")
+    # plain chain on the same frames, graph mode (the first call captures)
+    d = glsu.DPRA(phi0, params=params)
     d(frames[:10].contiguous(), out=out[:10], graph=True)
     d = glsu.DPRA(phi0, params=params)
     torch.cuda.synchronize()
@@ -87,61 +116,96 @@ def bench_learned(args, glsu, torch, np):
     d(frames, out=out, graph=True)
     torch.cuda.synchronize()
     plain_fps = T / (time.perf_counter() - t0)
-    # learned chain: per frame LS -> U-Net -> spot check (tile moves every frame)
-    def tile(t):
-        return ((t * 389) % (W - 64), (t * 613) % (H - 64))
+    del d
+    # learned chain (warm-up on a separate handle, then the timed one)
     dl = glsu.DPRA(phi0, params=params)
     for t in range(3):
-        dl.frame_learned(frames[t], net, out[t], tol=tol, tile=tile(t))
+        dl.frame_learned(frames[t], net, out[t], tol=tol, tile=_learned_tile(W, H, t))
         dl.update_ref_phi()
     dl = glsu.DPRA(phi0, params=params)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for t in range(T):  # asynchronous: the fallback decision stays on the device
-        dl.frame_learned(frames[t], net, out[t], tol=tol, tile=tile(t), sync=False)
+        dl.frame_learned(frames[t], net, out[t], tol=tol, tile=_learned_tile(W, H, t), sync=False)
         dl.update_ref_phi()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    learned_fps = T / dt
     st = dl.learned_stats()
-    # accuracy on sampled frames against the FP64 WFF2 of the same raw field
-    acc = {}
-    if not args.no_cpu:
-        from oracle import oracle as O
-        wf = O.WFF2(W, H, *wff, nthreads=min(os.cpu_count() or 1, 16))
-        dc = glsu.DPRA(phi0, params=params)
-        errs_net, errs_raw = [], []
-        sample = [0, T // 4, T // 2, T - 1]
-        for t in range(T):
-            dc.frame_learned(frames[t], net, tol=-1.0)
-            dc.update_ref_phi()
-            if t in sample:
-                z = dc.last_z().cpu().numpy().astype(np.complex128)
-                F = wf(z)
-                y = dc.last_net().cpu().numpy().astype(np.complex128)
-                en = np.angle(y * np.conj(F))
-                er = np.angle(z * np.conj(F))
-                errs_net.append(float(np.sqrt(np.mean(en * en))))
-                errs_raw.append(float(np.sqrt(np.mean(er * er))))
-        acc = {"frames": sample, "net_rms_rad": errs_net, "unfiltered_rms_rad": errs_raw,
-               "gain_over_unfiltered": round(float(np.mean(errs_raw) / np.mean(errs_net)), 2)}
-    line = {
-        "metric": METRIC, "value": round(learned_fps, 2), "unit": "frames/s", "n_gpus": 1,
-        "steps": 1, "warmup": 1, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (net, FP32 accumulate) + f32",
-        "data": "synthetic",
-        "config": {"workload": f"learned WFF2 surrogate, {W}x{H}, {T} frames: DPRA LS -> 4-level "
-                               "U-Net (32 base ch, BF16 MFMA) -> phase update, WFF2 on a "
-                               f"64x64 tile every frame, fallback above {tol} rad RMS; "
-                               f"noise std {noise:.2f}, amp0 {amp0:.2f}",
-                   "name": "learned1024", "weights": "trained in this run" if trained_here
-                   else os.path.relpath(path, ROOT)},
+    del dl
+    res = {
+        "workload": f"learned WFF2 surrogate, {W}x{H}, {T} frames: DPRA LS -> 4-level U-Net "
+                    "(32 base ch, BF16 MFMA, FP32 accumulate) -> phase update, WFF2 on a "
+                    f"64x64 tile every frame, fallback above {tol} rad RMS; noise std "
+                    f"{noise:.2f}, amp0 {amp0:.2f}",
+        "weights": os.path.relpath(path, ROOT),
+        "frames_per_s": round(T / dt, 2), "ms_per_frame": round(dt * 1e3 / T, 4),
         "plain_frames_per_s": round(plain_fps, 2),
-        "learned_over_plain": round(learned_fps / plain_fps, 3),
-        "fallback_frames": st["fallbacks"],
+        "learned_over_plain": round(plain_fps and (T / dt) / plain_fps, 3),
+        "fallback_frames": st["fallbacks"], "frames": T,
         "spot_check_rms_rad": {"mean": round(st["mean_rms"], 5), "max": round(st["max_rms"], 5)},
-        "accuracy_vs_fp64_wff2": acc,
     }
+    if not with_accuracy:
+        return res
+    from oracle import oracle as O
+    out_t = out.cpu().numpy()
+    sample = sorted(set([0, 1, 2] + list(range(25, T, 50)) + [T - 1]))
+    dc = glsu.DPRA(phi0, params=params)
+    out2 = torch.empty_like(out)
+    rows = []
+    for t in range(T):
+        r = dc.frame_learned(frames[t], net, out2[t], tol=tol, tile=_learned_tile(W, H, t),
+                             sync=t in sample)
+        if t in sample:
+            z = dc.last_z().cpu().numpy().astype(np.complex128)
+            y = dc.last_net().cpu().numpy().astype(np.complex128)
+            F = O.WFF2(W, H, *wff, nthreads=nthreads, pair_parallel=True)(z)
+            aF = np.angle(F)
+            o2 = out2[t].cpu().numpy().astype(np.float64)
+            inc = o2 if t == 0 else o2 - out2[t - 1].cpu().numpy().astype(np.float64)
+            rows.append({"frame": t, "fallback": bool(r[1]),
+                         "chain_rms_rad": round(O.wrapped_err(inc, aF)[0], 5),
+                         "net_rms_rad": round(O.wrapped_err(np.angle(y), aF)[0], 5),
+                         "unfiltered_rms_rad": round(O.wrapped_err(np.angle(z), aF)[0], 5)})
+        dc.update_ref_phi()
+    same = bool(np.array_equal(out2.cpu().numpy(), out_t))
+    ch = [r["chain_rms_rad"] for r in rows]
+    raw = [r["unfiltered_rms_rad"] for r in rows]
+    res["accuracy_vs_fp64_wff2"] = {
+        "note": "per sampled frame: chain = the timed chain's own filtered phase (increment of "
+                "its output; WFF2's where the frame fell back), net = the network's output on "
+                "the same raw field, unfiltered = the raw DPRA phase; all against the FP64 "
+                "WFF2 of that raw field",
+        "rerun_bitwise_equal_to_timed": same,
+        "chain_rms_rad_mean": round(float(np.mean(ch)), 5),
+        "chain_rms_rad_max": round(float(np.max(ch)), 5),
+        "gain_over_unfiltered": round(float(np.mean(raw) / max(np.mean(ch), 1e-30)), 2),
+        "frames": rows,
+    }
+    return res
+
+
+def bench_learned(args, glsu, torch, np):
+    """Learned filter vs plain WFF2 on one 1024^2 sequence: its own JSON line
+    (value = frames/s of the learned chain)."""
+    size, T, _, _ = WORKLOADS["learned1024"]
+    W = args.size or size
+    T = args.frames or T
+    hi = host_info()
+    res = learned_leg(args, glsu, torch, np, W=W, T=T, with_accuracy=not args.no_cpu,
+                      nthreads=hi["threads"])
+    if res is None:
+        print("bench.py: no committed U-Net weights (train them with python -m glsu.learned)",
+              file=sys.stderr)
+        sys.exit(1)
+    line = {
+        "metric": METRIC, "value": res["frames_per_s"], "unit": "frames/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": round(res["ms_per_frame"] * T, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 (net, FP32 accumulate) + f32", "data": "synthetic",
+        "config": {"workload": res.pop("workload"), "name": "learned1024",
+                   "weights": res.pop("weights")},
+    }
+    line.update(res)
     print(json.dumps(line), flush=True)
 
 
@@ -453,10 +517,16 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget for the FP64 CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--parity-frames", type=int, default=None,
+                    help="frames of the sequence checked against the FP64 oracle (default: all)")
+    ap.add_argument("--parity-floor", action="store_true",
+                    help="also run the oracle chain from phi0 rounded to float32")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="leave out the learned-surrogate object of the default line")
     ap.add_argument("--no-tune", action="store_true",
                     help="library default plans instead of the autotuned ones")
     ap.add_argument("--tuning", default=None,
-                    help="fixed tuning as planx,plany,cols_waves,rows_waves (skips the tuner)")
+                    help="fixed tuning as planx,plany,cols_waves,rows_waves[,ls_tile] (skips the tuner)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one plain copy kernel (torch.neg) over this many MiB of float32 "
                          "before the timed region, as the HBM counter calibration for "
@@ -494,8 +564,8 @@ def main():
     out = torch.empty((T, H, W), dtype=torch.float32, device="cuda")
     tuning = None
     if args.tuning:
-        px, py, cw, rw = (int(v) for v in args.tuning.split(","))
-        tuning = dict(planx=px, plany=py, cols_waves=cw, rows_waves=rw)
+        vals = [int(v) for v in args.tuning.split(",")]
+        tuning = dict(zip(("planx", "plany", "cols_waves", "rows_waves", "ls_tile"), vals))
     elif not args.no_tune:  # untimed: picks FFT plans and launch shapes for this size
         tuning = glsu.autotune.tuning_only(glsu.autotune.tune(
             W, H, params, force=True, log=lambda m: print(m, file=sys.stderr, flush=True)))
@@ -508,16 +578,37 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    # Every step is the whole sequence from its start: before each step (outside
+    # the timed interval) the handle is reset to phi_ref = phi0 and a zero
+    # cumulative phase, as a freshly constructed object holds them
+    # (DPRA/dpra_cpu.cpp:19, :39-43); the WFF threshold stays cached from the
+    # first frame, as the reference caches it (WFT/WFT2_CPU.cpp:1014-1015).
+    phr0, dpr0 = d.get_state()
+
+    def reset():
+        d.set_state(phr0, dpr0)
+        torch.cuda.synchronize()
+
+    # set-up, untimed: capture the chain's graphs (the first call also computes
+    # the threshold, so the second call captures the steady-state graph)
+    for _ in range(2):
+        reset()
+        d(frames, out=out, graph=graph)
     for _ in range(args.warmup):
+        reset()
         d(frames, out=out, graph=graph)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    step_s, step_ms = [], []
     for _ in range(args.steps):
-        d(frames, out=out, graph=graph)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+        reset()
+        t0 = time.perf_counter()
+        d(frames, out=out, graph=graph, timed=True)
+        torch.cuda.synchronize()
+        step_s.append(time.perf_counter() - t0)
+        step_ms.append(d.last_ms)
+    dt = sum(step_s)
     ms_per_step = dt * 1e3 / max(args.steps, 1)
     fps = args.steps * T / dt
+    gpu_out = out.cpu().numpy() if not args.no_cpu else None  # the last timed step's output
 
     # --- roofline of the dominant kernel (the column WFF2 kernel) ----------------
     # HIP events on the handle's stream around back-to-back launches of that
@@ -534,13 +625,13 @@ def main():
     np_ref = _opt(H + 2 * info["sy"]) * _opt(W + 2 * info["sx"])
     b_alg = 28 * W * H + 8 * np_ref * (1 + nwx * nwy)
     roofline = {
-        "bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
+        "bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
         "kernel": "wff_cols_kernel", "avg_ms": round(col_ms, 5),
+        "avg_ms_source": "HIP events around 20 back-to-back launches on the handle's stream",
         "note": "FFT butterflies on the FP32 vector ALU (no dense contraction, so no MFMA); "
-                "priced against the FP32 peak, which MI355X's matrix and vector units share; "
-                "that peak assumes packed FP32, which the kernels are compiled without (frac_unpacked "
-                "is against the unpacked vector rate)",
+                "priced against the FP32 peak, which assumes packed FP32, which the kernels "
+                "are compiled without (frac_unpacked is against the unpacked vector rate)",
         "frac_unpacked": round(achieved / PEAK_FP32_UNPACKED_TFLOPS, 4),
         "hbm_GBps": round(hbm, 1), "hbm_frac": round(hbm / PEAK_HBM_GBS, 4),
         "chain_hbm": {"B_alg_per_frame": b_alg, "GBps": round(b_alg * fps / 1e9, 1),
@@ -548,7 +639,11 @@ def main():
         "kernel_ms_per_frame": {k: round(v / 8, 5) for k, v in
                                 zip(["dpra_ls", "rows_fwd", "cols", "rows_inv_update"], kms)},
     }
-
+    prof = _profiled(W, H, d.tuning())
+    if prof is not None:
+        a_p = flops / (prof["avg_ms"] * 1e-3) / 1e12
+        roofline["rocprof"] = {"avg_ms": prof["avg_ms"], "achieved": round(a_p, 2),
+                               "frac": round(a_p / PEAK_FP32_TFLOPS, 4), "source": prof["source"]}
     traffic = _traffic(W, H, T, d.tuning())
     if traffic is not None:
         roofline["traffic"] = traffic["bytes"]
@@ -563,48 +658,115 @@ def main():
         "config": {"workload": f"plain FP32 DPRA+WFF2 chain, {W}x{H}, {T} frames per step, "
                                f"3x3 DPRA, WFF2 sigma={wff[0]:g}, wx,wy in [{wff[1]:g},{wff[2]:g}] "
                                f"step {wff[3]:g} ({nwx * nwy} pairs), thr=6*sqrt(mean|z|^2/3), "
-                               f"rr=1, noise {noise:g}, carrier {carrier:g}",
+                               f"rr=1, noise {noise:g}, carrier {carrier:g}; every step from "
+                               "phi0 and a zero cumulative phase",
                    "name": args.workload,
                    "height": H, "width": W, "frames_per_step": T, "pairs": nwx * nwy,
                    "fft_len": [info["nx"], info["ny"]], "mode": args.mode,
                    "tuning": d.tuning()},
+        "step_ms": {"wall": [round(v * 1e3, 3) for v in step_s],
+                    "hip_events": [round(v, 3) for v in step_ms]},
         "roofline": roofline,
     }
+    if args.workload == "heavy2048":
+        line["config"]["precision"] = (
+            "FP32 bulk arithmetic; the three sign-changing normal-equation sums accumulated "
+            "in FP64 (exact for the nine FP32 terms, in place of a compensated FP32 two-sum); "
+            "FP64 3x3 solve where |det A| < 1e-6 * A00*A11*A22")
+    del d
 
-    # --- CPU baseline (FP64 oracle) and parity on the same frames ---------------
+    # --- CPU baseline (FP64 oracle, the reference's loop structure) -------------
     if not args.no_cpu:
         from oracle import oracle as O
-        nthreads = min(os.cpu_count() or 1, 16)
-        host = frames.cpu().numpy().astype(np.float64)
+        hi = host_info()
+        nthreads = hi["threads"]
+        host = frames.cpu().numpy()
         od = O.DPRA(phi0, sigx=wff[0], wxl=wff[1], wxh=wff[2], wxi=wff[3], nthreads=nthreads)
-        cpu_out = []
         t0 = time.perf_counter()
         n = 0
         while n < T:
-            dphi, _ = od.dpra_per_frame(host[n])
+            od.dpra_per_frame(host[n].astype(np.float64))
             od.update_ref_phi()
-            cpu_out.append(dphi)
             n += 1
             if time.perf_counter() - t0 > args.cpu_seconds:
                 break
         cdt = time.perf_counter() - t0
-        gpu = glsu.DPRA(phi0, params=params, tuning=tuning)(frames[:n].contiguous()).cpu().numpy()
-        errs = []
-        for k in range(n):
-            e = np.angle(np.exp(1j * (gpu[k].astype(np.float64) - cpu_out[k])))
-            errs.append((float(np.sqrt(np.mean(e * e))), float(np.abs(e).max())))
+        del od
         line["cpu_baseline"] = {
             "value": round(n / cdt, 4), "unit": "frames/s", "cores": nthreads, "kind": "port",
-            "sample": f"first {n} frames of the same {W}x{H} sequence, FP64 oracle "
-                      f"(reference algorithm, OpenMP over wy: effective parallelism "
-                      f"min({nthreads}, {nwy}) in WFF2), {cdt:.1f} s",
+            "sample": f"first {n} frames of the same {W}x{H} sequence, FP64 oracle (the "
+                      f"reference algorithm on this project's CPU FFT; OpenMP over wy as "
+                      f"WFT/WFT2_CPU.cpp:227-237, effective WFF2 parallelism "
+                      f"min({nthreads}, {nwy}) = {min(nthreads, nwy)}), {cdt:.1f} s",
+            "host": hi,
         }
-        line["parity"] = {
-            "frames": n, "rms_rad_max_over_frames": max(e[0] for e in errs),
-            "max_abs_rad": max(e[1] for e in errs),
-            "per_frame_rms": [float("%.3e" % e[0]) for e in errs],
-        }
+        # --- parity: the whole sequence against the FP64 oracle, frame by frame --
+        # the FP64 oracle needs ~0.5 s per 1024^2 frame at 25 pairs on 16 threads, and
+        # about a minute per 2048^2 frame at 441 pairs: the heavy workload checks a
+        # leading prefix (valid: the chain is sequential)
+        default_nf = 2 if args.workload == "heavy2048" else T
+        nf = min(default_nf if args.parity_frames is None else args.parity_frames, T)
+        if nf > 0:
+            line["parity"] = chain_parity(np, O, phi0, host, gpu_out, nf, wff, nthreads,
+                                          floor=args.parity_floor)
+    if args.workload == "plain1024" and not args.no_learned:
+        line["learned"] = learned_leg(args, glsu, torch, np, with_accuracy=not args.no_cpu,
+                                      nthreads=host_info()["threads"])
     print(json.dumps(line), flush=True)
+
+
+def chain_parity(np, O, phi0, host, gpu_out, nf, wff, nthreads, floor=False):
+    """Frames 0..nf-1 of the GPU chain against the FP64 oracle chain from the
+    same phi0 (DPRA_CPU's loop, DPRA/dpra_cpu.cpp:58-89, rr = 1), frame by
+    frame: wrapped-phase RMS and max |error| in radians.  The oracle's WFF2
+    threads take (wy, wx) pairs so the 200-frame chain fits the run; with
+    floor=True the oracle chain is also run from phi0 rounded to float32 (the
+    GPU's storage precision) and its own distance is reported per frame."""
+    t0 = time.perf_counter()
+    kw = dict(sigx=wff[0], wxl=wff[1], wxh=wff[2], wxi=wff[3], nthreads=nthreads,
+              pair_parallel=True)
+    od = O.DPRA(phi0, **kw)
+    o32 = O.DPRA(phi0.astype(np.float32).astype(np.float64), **kw) if floor else None
+    rms, mx, fl = [], [], []
+    for t in range(nf):
+        f = host[t].astype(np.float64)
+        ref, _ = od.dpra_per_frame(f)
+        od.update_ref_phi()
+        r, m = O.wrapped_err(gpu_out[t], ref)
+        rms.append(r)
+        mx.append(m)
+        if o32 is not None:
+            r32, _ = o32.dpra_per_frame(f)
+            o32.update_ref_phi()
+            fl.append(O.wrapped_err(r32, ref)[0])
+        if t % 20 == 0:
+            print(f"parity frame {t}: rms {r:.3e} max {m:.3e}", file=sys.stderr, flush=True)
+    res = {
+        "frames": nf, "rms_rad_max_over_frames": max(rms), "max_abs_rad": max(mx),
+        "frames_over_1e-3_rms": int(sum(r > 1e-3 for r in rms)),
+        "per_frame_rms": [float("%.3e" % v) for v in rms],
+        "per_frame_max": [float("%.3e" % v) for v in mx],
+        "oracle": "FP64 chain from the exact phi0, WFF2 threads over (wy, wx) pairs",
+        "cpu_s": round(time.perf_counter() - t0, 1),
+    }
+    if fl:
+        res["oracle_floor_rms"] = [float("%.3e" % v) for v in fl]
+    return res
+
+
+def _profiled(W, H, tuning):
+    """Average column-kernel duration from the committed rocprofv3 kernel
+    trace summary, when it holds an entry for this size and tuning."""
+    path = os.path.join(ROOT, "profiles", f"wff_cols_{W}x{H}_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for e in t.get("entries", []):
+        if e.get("tuning") == tuning and e.get("size") == [H, W] and "trace_avg_ms" in e:
+            return {"avg_ms": float(e["trace_avg_ms"]), "source": os.path.relpath(path, ROOT)}
+    return None
 
 
 if __name__ == "__main__":

@@ -142,27 +142,7 @@ static bool reference_padding_ok(int W, int H, int sx, int sy) {
 
 // Row transforms (x axis) use one wavefront per transform (T == 64 plans);
 // column transforms may pack several narrower transforms per wavefront.
-static int default_plan(int need, bool t64_only) {
-  auto c = fft_candidates(need);
-  if (t64_only) {
-    std::vector<int> k;
-    for (int id : c)
-      for (auto& p : fft_plans())
-        if (p.id == id && p.T == 64) k.push_back(id);
-    c = k;
-  }
-  if (c.empty()) return -1;
-  // prefer one transform per wavefront unless a narrower plan is much shorter
-  int best64 = -1, bestN64 = 0, first = c[0], firstN = 0;
-  for (int id : c)
-    for (auto& p : fft_plans())
-      if (p.id == id) {
-        if (id == first) firstN = p.N;
-        if (p.T == 64 && best64 < 0) { best64 = id; bestN64 = p.N; }
-      }
-  if (best64 < 0) return first;
-  return (firstN < 0.8 * bestN64) ? first : best64;
-}
+static int default_plan(int need, bool t64_only) { return plan_default(need, t64_only); }
 
 // Host-only validation (no device needed): sizes, parameters, padding rule.
 static int validate(int W, int H, const glsu_wff_params* pr, int* planx, int* plany) {
@@ -196,6 +176,8 @@ static int validate_tuning(const glsu_tuning* t) {
     return fail(GLSU_ERR_ARG, "cols_waves must be 1, 2 or 4 (got %d)", t->cols_waves);
   if (t && !(t->rows_waves < 0 || (t->rows_waves >= 1 && t->rows_waves <= kRowWavesMax)))
     return fail(GLSU_ERR_ARG, "rows_waves must be in 1..%d (got %d)", kRowWavesMax, t->rows_waves);
+  if (t && t->ls_tile > 0 && !ls_tile_valid(t->ls_tile))
+    return fail(GLSU_ERR_ARG, "unknown least-squares tile shape %d", t->ls_tile);
   return GLSU_OK;
 }
 
@@ -208,6 +190,7 @@ static int plan_common(WffPlan& p, int W, int H, const glsu_wff_params* pr, cons
                          t ? t->rows_waves : -1, err);
   if (rc == -2) return fail(GLSU_ERR_ARG, "%s", err.c_str());
   if (rc) return fail(GLSU_ERR_DEVICE, "%s", err.c_str());
+  if (t && t->ls_tile > 0) p.ls_tile = t->ls_tile;
   return GLSU_OK;
 }
 
@@ -473,6 +456,7 @@ static void get_tuning(const WffPlan& p, glsu_tuning* t) {
   t->plany = p.plany;
   t->cols_waves = p.cols_waves;
   t->rows_waves = p.rows_waves;
+  t->ls_tile = p.ls_tile;
 }
 
 extern "C" int glsu_wft2_get_tuning(glsu_wft2* h, glsu_tuning* t) {
@@ -654,7 +638,7 @@ extern "C" int glsu_dpra_create_tuned(glsu_dpra** out, const double* phi0, int W
 static int dpra_enqueue(glsu_dpra* h, const void* frame, int dtype, float* d_dphi,
                         hipStream_t st) {
   WffPlan& p = h->p;
-  HIP_TRY(launch_dpra_ls(frame, dtype, h->phiref, h->z, h->W, h->H, h->W, st));
+  HIP_TRY(launch_dpra_ls(frame, dtype, h->phiref, h->z, h->W, h->H, h->W, st, h->p.ls_tile));
   if (!p.thr_ready) {
     HIP_TRY(launch_thr(p, h->z, st));
     p.thr_ready = true;
@@ -823,7 +807,7 @@ extern "C" int glsu_dpra_profile(glsu_dpra* h, const void* frames, int nframes, 
   for (int i = 0; i < nframes; ++i) {
     const char* f = (const char*)frames + (size_t)i * n * es;
     HIP_TRY(hipEventRecord(ev[0], h->s));
-    HIP_TRY(launch_dpra_ls(f, dtype, h->phiref, h->z, h->W, h->H, h->W, h->s));
+    HIP_TRY(launch_dpra_ls(f, dtype, h->phiref, h->z, h->W, h->H, h->W, h->s, h->p.ls_tile));
     if (!p.thr_ready) {
       HIP_TRY(launch_thr(p, h->z, h->s));
       p.thr_ready = true;
@@ -1290,7 +1274,7 @@ extern "C" int glsu_dpra_frame_learned(glsu_dpra* h, glsu_unet* net, const void*
     HIP_TRY(hipMemset(h->d_stats, 0, 5 * sizeof(float)));
     HIP_TRY(hipHostMalloc((void**)&h->h_res, 2 * sizeof(float)));
   }
-  HIP_TRY(launch_dpra_ls(frame, dtype, h->phiref, h->z, W, H, W, h->s));
+  HIP_TRY(launch_dpra_ls(frame, dtype, h->phiref, h->z, W, H, W, h->s, h->p.ls_tile));
   if (!p.thr_ready) {
     HIP_TRY(launch_thr(p, h->z, h->s));
     p.thr_ready = true;

@@ -7,36 +7,39 @@
 
 namespace glsu {
 
-// 64 x 8 pixel tiles (256-byte rows), one pixel per thread (DRPT rows per
-// thread is kept as a parameter: the per-pixel body branches, so several
-// pixels per thread serialise their long dependency chains and measured
-// slower), walked row-major and XCD-contiguous so that halo rows shared by
-// neighbouring tiles are fetched from HBM once per L2.
-constexpr int DTX = 64, DTY = 8, DRPT = 1;
-constexpr int DTHREADS = DTX * DTY / DRPT;
+// Pixel tiles of TX x TY (TX floats = one row segment), one pixel per thread
+// (several pixels per thread measured slower: the per-pixel body branches, so
+// their long dependency chains serialise), walked row-major and
+// XCD-contiguous so that halo rows shared by neighbouring tiles are fetched
+// from HBM once per L2.  The shape is a tuning choice (kLsTiles); the
+// per-pixel arithmetic does not depend on it.  64 x 8 is the default and the
+// batched kernel's shape.
+constexpr int DTX = 64, DTY = 8;
+constexpr int DTHREADS = DTX * DTY;
 
-// One DTX x DTY tile (bx, by) of one image; all threads of the block take part.
-template <typename InT>
+// One TX x TY tile (bx, by) of one image; all threads of the block take part.
+template <typename InT, int TX, int TY>
 __device__ __forceinline__ void dpra_ls_body(const InT* __restrict__ I,
                                              const float* __restrict__ phr,
                                              float2* __restrict__ z, int W, int H, int pitch,
                                              int bx, int by) {
-  __shared__ float sI[DTY + 2][DTX + 2];
-  __shared__ float sS[DTY + 2][DTX + 2];  // sin(phi/2)
-  __shared__ float sC[DTY + 2][DTX + 2];  // cos(phi/2)
-  const int tx = threadIdx.x % DTX, ty0 = (threadIdx.x / DTX) * DRPT;
-  const int x0 = bx * DTX - 1, y0 = by * DTY - 1;
-  // Staging of the (DTX+2) x (DTY+2) tile with its halo: KS slots per thread.
+  constexpr int NT = TX * TY;
+  __shared__ float sI[TY + 2][TX + 2];
+  __shared__ float sS[TY + 2][TX + 2];  // sin(phi/2)
+  __shared__ float sC[TY + 2][TX + 2];  // cos(phi/2)
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int x0 = bx * TX - 1, y0 = by * TY - 1;
+  // Staging of the (TX+2) x (TY+2) tile with its halo: KS slots per thread.
   // Every load is issued before any is used, from clamped (always in-bounds)
   // coordinates, and the out-of-image values are replaced afterwards: loads
   // under per-slot guards would each wait for the previous slot's round trip.
-  constexpr int NS = (DTX + 2) * (DTY + 2), KS = (NS + DTHREADS - 1) / DTHREADS;
+  constexpr int NS = (TX + 2) * (TY + 2), KS = (NS + NT - 1) / NT;
   float ld_i[KS], ld_p[KS];
   bool in[KS];
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
-    const int i = min((int)threadIdx.x + k * DTHREADS, NS - 1);
-    const int ly = i / (DTX + 2), lx = i % (DTX + 2);
+    const int i = min((int)threadIdx.x + k * NT, NS - 1);
+    const int ly = i / (TX + 2), lx = i % (TX + 2);
     const int gx = x0 + lx, gy = y0 + ly;
     in[k] = gx >= 0 && gx < W && gy >= 0 && gy < H;
     const int cx = min(max(gx, 0), W - 1), cy = min(max(gy, 0), H - 1);
@@ -45,9 +48,9 @@ __device__ __forceinline__ void dpra_ls_body(const InT* __restrict__ I,
   }
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
-    const int i = (int)threadIdx.x + k * DTHREADS;
+    const int i = (int)threadIdx.x + k * NT;
     if (i >= NS) break;
-    const int ly = i / (DTX + 2), lx = i % (DTX + 2);
+    const int ly = i / (TX + 2), lx = i % (TX + 2);
     float hs = 0.f, hc = 1.f;
     sincos_half(0.5f * ld_p[k], &hs, &hc);
     sI[ly][lx] = in[k] ? ld_i[k] : 0.f;
@@ -55,27 +58,20 @@ __device__ __forceinline__ void dpra_ls_body(const InT* __restrict__ I,
     sC[ly][lx] = in[k] ? hc : 1.f;
   }
   __syncthreads();
-  const int x = bx * DTX + tx;
-  if (x >= W) return;
-#pragma unroll
-  for (int r = 0; r < DRPT; ++r) {
-  const int ty = ty0 + r, y = by * DTY + ty;
-  if (y >= H) continue;
-
-  const float2 out =
-      dpra_pixel(&sI[0][0], &sS[0][0], &sC[0][0], DTX + 2, ty + 1, tx + 1, x, y, W, H);
-  z[(size_t)y * W + x] = out;
-  }  // rows of this thread
+  const int x = bx * TX + tx, y = by * TY + ty;
+  if (x >= W || y >= H) return;
+  z[(size_t)y * W + x] =
+      dpra_pixel(&sI[0][0], &sS[0][0], &sC[0][0], TX + 2, ty + 1, tx + 1, x, y, W, H);
 }
 
-template <typename InT>
-__global__ __launch_bounds__(DTHREADS) void dpra_ls_kernel(const InT* __restrict__ I,
-                                                           const float* __restrict__ phr,
-                                                           float2* __restrict__ z, int W, int H,
-                                                           int pitch) {
-  const int nbx = (W + DTX - 1) / DTX;
+template <typename InT, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void dpra_ls_kernel(const InT* __restrict__ I,
+                                                          const float* __restrict__ phr,
+                                                          float2* __restrict__ z, int W, int H,
+                                                          int pitch) {
+  const int nbx = (W + TX - 1) / TX;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  dpra_ls_body<InT>(I, phr, z, W, H, pitch, tile % nbx, tile / nbx);
+  dpra_ls_body<InT, TX, TY>(I, phr, z, W, H, pitch, tile % nbx, tile / nbx);
 }
 
 // Batched form: persistent walk over the 64 x 8 tiles of every image in the
@@ -99,8 +95,8 @@ __global__ __launch_bounds__(DTHREADS) void dpra_ls_batch_kernel(const BatchSeg*
     const int lt = tile - s.tile0;
     const int c = (t == 0 ? 0 : (t - 1) / s.rr + 1) & 1;
     __syncthreads();  // LDS of the previous tile is free
-    dpra_ls_body<InT>((const InT*)s.frames + (size_t)t * s.W * s.H, s.phi[c], s.z, s.W, s.H, s.W,
-                      lt % nbx, lt / nbx);
+    dpra_ls_body<InT, DTX, DTY>((const InT*)s.frames + (size_t)t * s.W * s.H, s.phi[c], s.z, s.W,
+                                s.H, s.W, lt % nbx, lt / nbx);
   }
 }
 
@@ -200,26 +196,54 @@ int batch_grid_ls(int ntiles) {
 
 int dpra_ls_tiles(int W, int H) { return ((W + DTX - 1) / DTX) * ((H + DTY - 1) / DTY); }
 
-hipError_t launch_dpra_ls(const void* frame, int dtype, const float* phiref, float2* z, int W,
-                          int H, int pitch, hipStream_t s) {
-  dim3 grid(((W + DTX - 1) / DTX) * ((H + DTY - 1) / DTY)), block(DTHREADS);
+// Least-squares tile shapes (tuning ids 1..kLsTileCount; 1 = 64 x 8).
+#define GLSU_LS_TILES(X) X(1, 64, 8) X(2, 64, 4) X(3, 32, 8) X(4, 128, 4) X(5, 64, 16) X(6, 32, 16)
+
+bool ls_tile_valid(int id) {
+  switch (id) {
+#define GLSU_LS_VALID(i, tx, ty) case i:
+    GLSU_LS_TILES(GLSU_LS_VALID)
+#undef GLSU_LS_VALID
+    return true;
+    default:
+      return false;
+  }
+}
+
+template <int TX, int TY>
+static hipError_t launch_ls_shape(const void* frame, int dtype, const float* phiref, float2* z,
+                                  int W, int H, int pitch, hipStream_t s) {
+  dim3 grid(((W + TX - 1) / TX) * ((H + TY - 1) / TY)), block(TX * TY);
   switch (dtype) {
     case IN_U8:
-      hipLaunchKernelGGL(dpra_ls_kernel<uint8_t>, grid, block, 0, s, (const uint8_t*)frame,
-                         phiref, z, W, H, pitch);
+      hipLaunchKernelGGL((dpra_ls_kernel<uint8_t, TX, TY>), grid, block, 0, s,
+                         (const uint8_t*)frame, phiref, z, W, H, pitch);
       break;
     case IN_F32:
-      hipLaunchKernelGGL(dpra_ls_kernel<float>, grid, block, 0, s, (const float*)frame, phiref,
-                         z, W, H, pitch);
+      hipLaunchKernelGGL((dpra_ls_kernel<float, TX, TY>), grid, block, 0, s, (const float*)frame,
+                         phiref, z, W, H, pitch);
       break;
     case IN_F64:
-      hipLaunchKernelGGL(dpra_ls_kernel<double>, grid, block, 0, s, (const double*)frame,
-                         phiref, z, W, H, pitch);
+      hipLaunchKernelGGL((dpra_ls_kernel<double, TX, TY>), grid, block, 0, s,
+                         (const double*)frame, phiref, z, W, H, pitch);
       break;
     default:
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_dpra_ls(const void* frame, int dtype, const float* phiref, float2* z, int W,
+                          int H, int pitch, hipStream_t s, int tile) {
+  switch (tile) {
+#define GLSU_LS_CASE(i, tx, ty) \
+  case i:                       \
+    return launch_ls_shape<tx, ty>(frame, dtype, phiref, z, W, H, pitch, s);
+    GLSU_LS_TILES(GLSU_LS_CASE)
+#undef GLSU_LS_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
 }
 
 // ---------------------------------------------------------------------------

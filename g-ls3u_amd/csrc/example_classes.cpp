@@ -3,6 +3,11 @@
 //   AIA::AIA_HIPF(M, W, H) and operator()(phi, deltas, time, iters, err,
 //     frames, maxIter, maxErr) on M phase-shifted 8-bit frames
 //     (App_DPRA/dpraworker.cpp:21-57 calls the CPU twin the same way);
+//   WFT_FPA::WFT::WFT2_HIPF in WFF mode: the constructor allocates
+//     WFT2_DeviceResultsF::m_d_filtered and the struct's destructor frees it
+//     (WFT/WFT2_CUDAf.h:21-40, WFT/WFT.cpp:118-130); checked on a plane wave
+//     whose frequency is on the (wx, wy) grid, where the filtered phase equals
+//     the input phase at pixels 2s or more from the border;
 //   WFT_FPA::WFT::WFT2_HIPF in WFR mode with explicit parameters, filling
 //     WFT2_DeviceResultsF's ridge maps (WFT/WFT2_CUDAf.h:21-40).
 // Prints one line per check with the measured error; exits 0 when all pass.
@@ -47,6 +52,48 @@ static int run_aia() {
   return (dmax < 0.02 && prms < 0.02) ? 0 : 1;
 }
 
+static int run_wff() {
+  const int W = 96, H = 80, s = 12;  // sigma 4 -> half window round(3 sigma) = 12
+  const double wx0 = 0.1, wy0 = -0.2;
+  std::vector<float2> f((size_t)W * H);
+  for (int i = 0; i < H; ++i)
+    for (int j = 0; j < W; ++j) {
+      const double p = wx0 * j + wy0 * i - 0.7;
+      f[(size_t)i * W + j] = make_float2((float)std::cos(p), (float)std::sin(p));
+    }
+  float2* d_f = nullptr;
+  if (hipMalloc((void**)&d_f, sizeof(float2) * f.size()) != hipSuccess) return 1;
+  (void)hipMemcpy(d_f, f.data(), sizeof(float2) * f.size(), hipMemcpyHostToDevice);
+  float2* filtered = nullptr;
+  int attr_rc = -1, mem_type = -1;
+  std::vector<float2> F(f.size());
+  double t = 0;
+  {
+    WFT_FPA::WFT::WFT2_DeviceResultsF z;
+    WFT_FPA::WFT::WFT2_HIPF wff(W, H, WFT_FPA::WFT::WFT_TYPE::WFF, 4.0, -0.2, 0.2, 0.1, 4.0,
+                                -0.2, 0.2, 0.1, -1, z);
+    filtered = z.m_d_filtered;  // allocated by the constructor
+    hipPointerAttribute_t a;
+    attr_rc = (int)hipPointerGetAttributes(&a, filtered);
+    mem_type = (int)a.type;
+    wff(d_f, z, t);
+    (void)hipMemcpy(F.data(), z.m_d_filtered, sizeof(float2) * F.size(), hipMemcpyDeviceToHost);
+  }  // ~WFT2_HIPF, then ~WFT2_DeviceResultsF frees m_d_filtered
+  (void)hipFree(d_f);
+  double ep = 0, amin = 1e30;
+  for (int i = 2 * s; i < H - 2 * s; ++i)
+    for (int j = 2 * s; j < W - 2 * s; ++j) {
+      const size_t q = (size_t)i * W + j;
+      const double ph = std::atan2((double)F[q].y, (double)F[q].x);
+      ep = std::fmax(ep, std::fabs(wrap(ph - (wx0 * j + wy0 * i - 0.7))));
+      amin = std::fmin(amin, std::hypot((double)F[q].x, (double)F[q].y));
+    }
+  const bool allocated = filtered != nullptr && attr_rc == 0 && mem_type == (int)hipMemoryTypeDevice;
+  std::printf("wff: %.3f ms; filtered field allocated by the constructor: %s; interior phase "
+              "error %.2e, min |F| %.3f\n", t, allocated ? "yes" : "no", ep, amin);
+  return (allocated && ep < 1e-5 && amin > 0.5) ? 0 : 1;
+}
+
 static int run_wfr() {
   const int W = 96, H = 80;
   const double wx0 = 0.2, wy0 = -0.1;
@@ -86,9 +133,10 @@ static int run_wfr() {
 int main() {
   try {
     const int a = run_aia();
+    const int f = run_wff();
     const int w = run_wfr();
-    std::printf("example_classes: %s\n", (a | w) ? "FAILED" : "ok");
-    return a | w;
+    std::printf("example_classes: %s\n", (a | f | w) ? "FAILED" : "ok");
+    return a | f | w;
   } catch (int e) {
     std::printf("example_classes: constructor threw %d\n", e);
     return 2;

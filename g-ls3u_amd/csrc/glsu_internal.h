@@ -30,6 +30,7 @@ struct WffPlan {
   int planx = -1, plany = -1;  // FFT plan ids along x (rows) and y (columns)
   int Nx = 0, Ny = 0;
   int cols_waves = 4, rows_waves = 1;  // launch shapes
+  int ls_tile = 1;                      // DPRA least-squares tile shape (launch_dpra_ls)
   // device buffers
   // window spectra / N, lane-major: [w][j][lane] = bin kbase(lane) + koff(j).
   // Real: the window is even, so its modulated spectrum has no imaginary part.
@@ -82,6 +83,17 @@ int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh,
                   int plany, int cols_waves, int rows_waves, std::string& err,
                   bool buffers = true);  // false: spectra/twiddle tables only
 void wff_plan_free(WffPlan& p);
+// The scalar part of wff_plan_init (sizes, grid, plans, launch shapes) with
+// no allocation of any kind; *err names the failure.
+int wff_plan_geometry(WffPlan& p, int W, int H, double sigx, double wxl, double wxh, double wxi,
+                      double sigy, double wyl, double wyh, double wyi, double thr, int planx,
+                      int plany, int cols_waves, int rows_waves, const char** err);
+// Default plan for a padded length >= need (row plans: t64_only).
+int plan_default(int need, bool t64_only);
+// wff_plan_init's tables (windows, spectra, twiddles; the threshold when
+// thr_param >= 0) computed on the device into p.d_Hx, d_Hy, d_twx, d_twy,
+// d_thr; gx, gy: device scratch of 2 sx + 1 and 2 sy + 1 doubles.
+hipError_t launch_plan_tables(const WffPlan& p, double* gx, double* gy, hipStream_t s);
 
 // Kernel launchers (all asynchronous on `s`).
 hipError_t launch_thr(const WffPlan& p, const float2* f, hipStream_t s);
@@ -203,8 +215,10 @@ hipError_t launch_frame_advance(int* frame, hipStream_t s);
 
 // DPRA least squares: z = exp(j*Delta) from frame + phiref
 enum InDtype { IN_U8 = 0, IN_F32 = 1, IN_F64 = 2 };
+// tile: least-squares tile shape id (1 = 64 x 8 ... ; ls_tile_valid)
 hipError_t launch_dpra_ls(const void* frame, int dtype, const float* phiref, float2* z, int W,
-                          int H, int pitch, hipStream_t s);
+                          int H, int pitch, hipStream_t s, int tile = 1);
+bool ls_tile_valid(int id);
 // synthetic fringe frames (device generator, same definition as the oracle's)
 hipError_t launch_gen_frames(float* out, int W, int H, int t0, int T, uint64_t seed, double amp0,
                              double amp1, double carrier, double noise, hipStream_t s);

@@ -42,6 +42,19 @@ namespace glsu {
   X(13, 32, 12, 4, 4, 2)                 \
   X(14, 32, 20, 4, 4, 2)
 
+// The same table as constants (no allocation at any call).
+struct PlanRow {
+  int id, T, E, N;
+};
+#define GLSU_PLAN_ROW(id, T, E, ...) {id, T, E, T * E},
+inline constexpr PlanRow kPlanRows[] = {GLSU_PLANS(GLSU_PLAN_ROW)};
+#undef GLSU_PLAN_ROW
+inline const PlanRow* plan_row(int id) {
+  for (const PlanRow& r : kPlanRows)
+    if (r.id == id) return &r;
+  return nullptr;
+}
+
 __device__ __forceinline__ size_t tile_index(int y, int x, int H) {
   return ((size_t)(x / kTileCols) * H + y) * kTileCols + (x % kTileCols);
 }
@@ -478,13 +491,12 @@ static int plan_lds(int id) {
   }
 }
 
-static int plan_smem_bytes(int id, int waves, bool ttab = false) {
+[[maybe_unused]] static int plan_smem_bytes(int id, int waves, bool ttab = false) {
   // per transform: the FftPlan::LDS exchange buffer, plus the shared E x T
   // transition-twiddle table when the kernel keeps it in LDS
-  for (auto& p : fft_plans())
-    if (p.id == id)
-      return (int)sizeof(float2) * ((waves * 64 / p.T) * plan_lds(id) + (ttab ? p.E * p.T : 0));
-  return 0;
+  const PlanRow* p = plan_row(id);
+  if (!p) return 0;
+  return (int)sizeof(float2) * ((waves * 64 / p->T) * plan_lds(id) + (ttab ? p->E * p->T : 0));
 }
 
 template <class P>

@@ -117,10 +117,27 @@ hipError_t launch_frame_advance(int* frame, hipStream_t s) {
 }
 
 // --------------------------------------------------------- plan set-up
-static const FftPlanInfo* find_plan(int id) {
-  for (auto& p : fft_plans())
-    if (p.id == id) return &p;
-  return nullptr;
+// Smallest compiled plan with N >= need, in (N, id) order (the first of
+// fft_candidates(need)); t64_only: plans with one transform per wavefront.
+static int smallest_plan(int need, bool t64_only) {
+  int best = -1, bestN = 0;
+  for (const PlanRow& r : kPlanRows) {
+    if (r.N < need || (t64_only && r.T != 64)) continue;
+    if (best < 0 || r.N < bestN || (r.N == bestN && r.id < best)) {
+      best = r.id;
+      bestN = r.N;
+    }
+  }
+  return best;
+}
+
+int plan_default(int need, bool t64_only) {
+  // prefer one transform per wavefront unless a narrower plan is much shorter
+  const int first = smallest_plan(need, t64_only);
+  const int best64 = smallest_plan(need, true);
+  if (first < 0) return -1;
+  if (best64 < 0) return first;
+  return (plan_row(first)->N < 0.8 * plan_row(best64)->N) ? first : best64;
 }
 
 // Window spectra in the order the kernels read them: entry (r, j, lane) holds
@@ -191,6 +208,86 @@ static std::vector<float2> twiddles(int N) {
   return t;
 }
 
+// ---------------------------------------------------- tables on the device
+// The tables wff_plan_init builds on the host (normalised 1-D windows, window
+// spectra in lane-major order, twiddles), computed by kernels in FP64 with the
+// host's expressions in the host's order, into memory the caller provides.
+// The FP64 exp/cos of the device and of the host C library can differ in the
+// last bit, so a table entry can, rarely, round to the neighbouring float.
+__global__ void norm_window_kernel(double sig, int s, double* __restrict__ g) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;  // serial sum, host order
+  double n2 = 0;
+  for (int d = -s; d <= s; ++d) {
+    const double v = exp(-(double)(d * d) / 2 / sig / sig);
+    g[d + s] = v;
+    n2 += v * v;
+  }
+  const double inv = 1.0 / sqrt(n2);
+  for (int i = 0; i < 2 * s + 1; ++i) g[i] *= inv;
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void spectra_kernel(const double* __restrict__ g, int s,
+                                                      double wl, double wi, int rows,
+                                                      float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * P::N) return;
+  const int r = i / P::N, q = i % P::N;
+  const int k = P::kbase(q % P::T) + P::koff(q / P::T);
+  const double w = wl + (double)r * wi;
+  double re = 0;
+  for (int d = -s; d <= s; ++d) {
+    long kd = ((long)k * d) % P::N;
+    if (kd < 0) kd += P::N;
+    const double a = w * d - 2.0 * M_PI * (double)kd / (double)P::N;
+    re += g[d + s] * cos(a);
+  }
+  out[i] = (float)(re / P::N);
+}
+
+__global__ __launch_bounds__(256) void twiddle_kernel(int N, float2* __restrict__ t) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  const double a = 2.0 * M_PI * (double)k / (double)N;
+  t[k] = make_float2((float)cos(a), (float)-sin(a));
+}
+
+__global__ void set_f32_kernel(float* __restrict__ p, float v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
+template <class P>
+static void launch_spectra(const double* g, int s, double wl, double wi, int rows, float* out,
+                           hipStream_t st) {
+  const int n = rows * P::N;
+  hipLaunchKernelGGL(spectra_kernel<P>, dim3((n + 255) / 256), dim3(256), 0, st, g, s, wl, wi,
+                     rows, out);
+}
+
+hipError_t launch_plan_tables(const WffPlan& p, double* gx, double* gy, hipStream_t st) {
+  hipLaunchKernelGGL(norm_window_kernel, dim3(1), dim3(64), 0, st, p.sigx, p.sx, gx);
+  hipLaunchKernelGGL(norm_window_kernel, dim3(1), dim3(64), 0, st, p.sigy, p.sy, gy);
+  switch (p.planx) {
+#define BODY launch_spectra<P>(gx, p.sx, p.wxl, p.wxi, p.nwx, p.d_Hx, st)
+    GLSU_PLANS(GLSU_DISPATCH_CASE)
+#undef BODY
+    default:
+      return hipErrorInvalidValue;
+  }
+  switch (p.plany) {
+#define BODY launch_spectra<P>(gy, p.sy, p.wyl, p.wyi, p.nwy, p.d_Hy, st)
+    GLSU_PLANS(GLSU_DISPATCH_CASE)
+#undef BODY
+    default:
+      return hipErrorInvalidValue;
+  }
+  hipLaunchKernelGGL(twiddle_kernel, dim3((p.Nx + 255) / 256), dim3(256), 0, st, p.Nx, p.d_twx);
+  hipLaunchKernelGGL(twiddle_kernel, dim3((p.Ny + 255) / 256), dim3(256), 0, st, p.Ny, p.d_twy);
+  if (p.thr_param >= 0)
+    hipLaunchKernelGGL(set_f32_kernel, dim3(1), dim3(64), 0, st, p.d_thr, (float)p.thr_param);
+  return hipGetLastError();
+}
+
 template <class V>
 static hipError_t upload(V*& dptr, const std::vector<V>& h) {
   hipError_t e = hipMalloc((void**)&dptr, sizeof(V) * h.size());
@@ -198,10 +295,9 @@ static hipError_t upload(V*& dptr, const std::vector<V>& h) {
   return hipMemcpy(dptr, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice);
 }
 
-int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh, double wxi,
-                  double sigy, double wyl, double wyh, double wyi, double thr, int planx,
-                  int plany, int cols_waves, int rows_waves, std::string& err,
-                  bool buffers) {
+int wff_plan_geometry(WffPlan& p, int W, int H, double sigx, double wxl, double wxh, double wxi,
+                      double sigy, double wyl, double wyh, double wyi, double thr, int planx,
+                      int plany, int cols_waves, int rows_waves, const char** err) {
   p = WffPlan{};
   p.W = W;
   p.H = H;
@@ -215,25 +311,23 @@ int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh,
   p.nwx = (int)((wxh - wxl) * (1 / wxi)) + 1;
   p.nwy = (int)((wyh - wyl) * (1 / wyi)) + 1;
   if (p.nwx < 1 || p.nwy < 1) {
-    err = "empty frequency grid";
+    *err = "empty frequency grid";
     return -1;
   }
   p.scale = (float)(0.25 * (1.0 / (M_PI * M_PI)) * wxi * wyi);
   const int needx = W + p.sx, needy = H + p.sy;
   if (planx < 0) {
-    auto c = fft_candidates(needx);
-    if (c.empty()) { err = "no FFT plan long enough along x"; return -2; }
-    planx = c[0];
+    planx = smallest_plan(needx, false);
+    if (planx < 0) { *err = "no FFT plan long enough along x"; return -2; }
   }
   if (plany < 0) {
-    auto c = fft_candidates(needy);
-    if (c.empty()) { err = "no FFT plan long enough along y"; return -2; }
-    plany = c[0];
+    plany = smallest_plan(needy, false);
+    if (plany < 0) { *err = "no FFT plan long enough along y"; return -2; }
   }
-  const FftPlanInfo* px = find_plan(planx);
-  const FftPlanInfo* py = find_plan(plany);
+  const PlanRow* px = plan_row(planx);
+  const PlanRow* py = plan_row(plany);
   if (!px || !py || px->N < needx || py->N < needy) {
-    err = "FFT plan too short for image + half window";
+    *err = "FFT plan too short for image + half window";
     return -2;
   }
   p.planx = planx; p.plany = plany;
@@ -247,6 +341,19 @@ int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh,
   while (p.rows_waves > 1 && plan_smem_bytes(planx, p.rows_waves) > 65536) --p.rows_waves;
   const int wb = (W + kTileCols - 1) / kTileCols;
   p.plane_elems = (size_t)wb * kTileCols * H;
+  return 0;
+}
+
+int wff_plan_init(WffPlan& p, int W, int H, double sigx, double wxl, double wxh, double wxi,
+                  double sigy, double wyl, double wyh, double wyi, double thr, int planx,
+                  int plany, int cols_waves, int rows_waves, std::string& err,
+                  bool buffers) {
+  const char* gerr = nullptr;
+  if (int rc = wff_plan_geometry(p, W, H, sigx, wxl, wxh, wxi, sigy, wyl, wyh, wyi, thr, planx,
+                                 plany, cols_waves, rows_waves, &gerr)) {
+    err = gerr;
+    return rc;
+  }
   // spectra tables
   auto gx = norm_window(sigx, p.sx), gy = norm_window(sigy, p.sy);
   std::vector<float> hx, hy;

@@ -48,14 +48,18 @@ class WfrResults(ctypes.Structure):
                 ("wx", "wy", "phase", "phase_comp", "b", "r", "cxx", "cyy")]
 
 
+TUNING_FIELDS = ("planx", "plany", "cols_waves", "rows_waves", "ls_tile")
+LS_TILES = {1: (64, 8), 2: (64, 4), 3: (32, 8), 4: (128, 4), 5: (64, 16), 6: (32, 16)}
+
+
 class Tuning(ctypes.Structure):
-    """glsu_tuning: plan ids and launch shapes (-1 = library default)."""
-    _fields_ = [("planx", ctypes.c_int), ("plany", ctypes.c_int),
-                ("cols_waves", ctypes.c_int), ("rows_waves", ctypes.c_int)]
+    """glsu_tuning: plan ids and launch shapes (-1 = library default);
+    ls_tile: least-squares tile shape id (LS_TILES)."""
+    _fields_ = [(f, ctypes.c_int) for f in TUNING_FIELDS]
 
     @staticmethod
-    def make(planx=-1, plany=-1, cols_waves=-1, rows_waves=-1):
-        return Tuning(int(planx), int(plany), int(cols_waves), int(rows_waves))
+    def make(planx=-1, plany=-1, cols_waves=-1, rows_waves=-1, ls_tile=-1):
+        return Tuning(int(planx), int(plany), int(cols_waves), int(rows_waves), int(ls_tile))
 
 
 class SeqDesc(ctypes.Structure):
@@ -68,10 +72,9 @@ def tuning_arg(tuning=None, planx=-1, plany=-1):
     """Tuning struct from a Tuning, a dict with any of its fields, or None
     (planx / plany given separately override the dict)."""
     if isinstance(tuning, Tuning):
-        t = Tuning.make(tuning.planx, tuning.plany, tuning.cols_waves, tuning.rows_waves)
+        t = Tuning.make(*(getattr(tuning, f) for f in TUNING_FIELDS))
     else:
-        t = Tuning.make(**{k: v for k, v in (tuning or {}).items()
-                           if k in ("planx", "plany", "cols_waves", "rows_waves")})
+        t = Tuning.make(**{k: v for k, v in (tuning or {}).items() if k in TUNING_FIELDS})
     if planx >= 0:
         t.planx = int(planx)
     if plany >= 0:
@@ -80,7 +83,7 @@ def tuning_arg(tuning=None, planx=-1, plany=-1):
 
 
 def tuning_dict(t):
-    return dict(planx=t.planx, plany=t.plany, cols_waves=t.cols_waves, rows_waves=t.rows_waves)
+    return {f: getattr(t, f) for f in TUNING_FIELDS}
 
 
 class WffParams(ctypes.Structure):
@@ -183,6 +186,22 @@ def lib():
         fn.argtypes = args
     _lib = L
     return L
+
+
+def check_out(t, shape, dtype, name="out"):
+    """Validate a caller-supplied output tensor before its address goes to the
+    C ABI: a contiguous CUDA tensor of exactly `shape` and `dtype` (the
+    library writes shape-many elements through the raw pointer).  Raises
+    ValueError otherwise; returns t."""
+    if not getattr(t, "is_cuda", False):
+        raise ValueError(f"{name} must be a CUDA tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t
 
 
 def check(rc):

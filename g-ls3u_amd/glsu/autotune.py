@@ -9,7 +9,10 @@ rounding; the choice only affects speed.
 
 The search is staged because the kernels are independent:
   1. columns: (plany, cols_waves), timed on the column kernel alone;
-  2. rows:    (planx, rows_waves), timed on the forward + inverse row kernels.
+  2. rows:    (planx, rows_waves), timed on the forward + inverse row kernels;
+  3. DPRA least squares: the pixel tile shape (ls_tile), timed on that kernel
+     (the reference fixes its DPRA blocks, DPRA/dpra_cudaf.cu:121-122; the
+     shape does not change the result).
 Timings come from glsu_dpra_profile (HIP events between kernels) over a few
 synthetic frames of the requested size, taken at three points of a synthetic
 sequence with the chain's own state there (the times are summed; "ms" in the
@@ -22,7 +25,7 @@ import os
 import numpy as np
 import torch
 
-from ._lib import fft_plans
+from ._lib import LS_TILES, TUNING_FIELDS, fft_plans
 from .dpra import DPRA, gen_frames
 from .params import wff_params
 from . import synth
@@ -121,7 +124,7 @@ def _samples(width, height, frames, params):
 
 
 def tune(width, height, params=None, frames=4, reps=3, cache=None, force=False, log=None):
-    """Pick (planx, plany, cols_waves, rows_waves) for a width x height WFF2 /
+    """Pick (planx, plany, cols_waves, rows_waves, ls_tile) for a width x height WFF2 /
     DPRA with `params`.  Returns a dict usable as `tuning=` for glsu.DPRA and
     glsu.WFT2, with the per-frame kernel times it measured (key "ms")."""
     params = params if params is not None else wff_params()
@@ -157,6 +160,14 @@ def tune(width, height, params=None, frames=4, reps=3, cache=None, force=False, 
             if best_r is None or t < best_r[0]:
                 best_r = (t, px, rw)
     tuning = dict(planx=best_r[1], plany=best_c[1], cols_waves=best_c[2], rows_waves=best_r[2])
+    # 3. least-squares tile shape
+    best_l = None
+    for lt, shape in LS_TILES.items():
+        _, ms = _time(samples, params, dict(tuning, ls_tile=lt), reps)
+        say(f"least squares tile {shape[0]}x{shape[1]}: {ms[0] * 1e3:.1f} us")
+        if best_l is None or ms[0] < best_l[0]:
+            best_l = (ms[0], lt)
+    tuning["ls_tile"] = best_l[1]
     _, ms = _time(samples, params, tuning, reps)
     res = dict(tuning, ms=ms)
     if path:
@@ -167,5 +178,6 @@ def tune(width, height, params=None, frames=4, reps=3, cache=None, force=False, 
 
 
 def tuning_only(res):
-    """Strip measurement fields from a tune() result."""
-    return {k: res[k] for k in ("planx", "plany", "cols_waves", "rows_waves")}
+    """Strip measurement fields from a tune() result (fields a cached result
+    predates keep the library default)."""
+    return {k: res.get(k, -1) for k in TUNING_FIELDS}

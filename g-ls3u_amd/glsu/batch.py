@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import SeqDesc, Tuning, check, lib, tuning_arg, tuning_dict
+from ._lib import SeqDesc, Tuning, check, check_out, lib, tuning_arg, tuning_dict
 from .params import wff_params
 
 _DT = {torch.uint8: 0, torch.float32: 1, torch.float64: 2}
@@ -81,6 +81,10 @@ class DPRABatch:
         if outs is None:
             outs = [torch.empty((T, H, W), dtype=torch.float32, device=frames[0].device)
                     for (H, W) in self.shapes]
+        if len(outs) != self.n:
+            raise ValueError(f"expected {self.n} output tensors")
+        for i, (o, (H, W)) in enumerate(zip(outs, self.shapes)):
+            check_out(o, (T, H, W), torch.float32, f"outs[{i}]")
         fp = (ctypes.c_void_p * self.n)(*[f.data_ptr() for f in frames])
         op = (ctypes.c_void_p * self.n)(*[o.data_ptr() for o in outs])
         ms = ctypes.c_double()

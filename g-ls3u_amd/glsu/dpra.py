@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import F32, F64, U8, Tuning, check, lib, tuning_arg, tuning_dict
+from ._lib import F32, F64, U8, Tuning, check, check_out, lib, tuning_arg, tuning_dict
 from .params import wff_params
 
 _DT = {torch.uint8: U8, torch.float32: F32, torch.float64: F64}
@@ -69,8 +69,12 @@ class DPRA:
     def dpra_per_frame(self, frame, dphi=None, phicurr=None, timed=False):
         """One frame: LS + WFF2 + phase update.  Returns dphi (float32 CUDA)."""
         dt = self._check_frame(frame)
+        hw = (self.height, self.width)
         if dphi is None:
-            dphi = torch.empty((self.height, self.width), dtype=torch.float32, device=frame.device)
+            dphi = torch.empty(hw, dtype=torch.float32, device=frame.device)
+        check_out(dphi, hw, torch.float32, "dphi")
+        if phicurr is not None:
+            check_out(phicurr, hw, torch.float32, "phicurr")
         ms = ctypes.c_double()
         check(lib().glsu_dpra_frame(self._h, ctypes.c_void_p(frame.data_ptr()), dt,
                                     ctypes.c_void_p(dphi.data_ptr()),
@@ -86,8 +90,10 @@ class DPRA:
         """frames: (T, H, W) CUDA tensor.  Returns dPhi_Sum as (T, H, W) float32."""
         dt = self._check_frame(frames, nd=3)
         T = frames.shape[0]
+        shape = (T, self.height, self.width)
         if out is None:
-            out = torch.empty((T, self.height, self.width), dtype=torch.float32, device=frames.device)
+            out = torch.empty(shape, dtype=torch.float32, device=frames.device)
+        check_out(out, shape, torch.float32)
         ms = ctypes.c_double()
         fn = lib().glsu_dpra_run_graph if graph else lib().glsu_dpra_run
         check(fn(self._h, ctypes.c_void_p(frames.data_ptr()), int(T), dt,
@@ -100,8 +106,10 @@ class DPRA:
         device milliseconds of (least squares, rows fwd, columns, rows inv)."""
         dt = self._check_frame(frames, nd=3)
         T = frames.shape[0]
+        shape = (T, self.height, self.width)
         if out is None:
-            out = torch.empty((T, self.height, self.width), dtype=torch.float32, device=frames.device)
+            out = torch.empty(shape, dtype=torch.float32, device=frames.device)
+        check_out(out, shape, torch.float32)
         ms = (ctypes.c_double * 4)()
         check(lib().glsu_dpra_profile(self._h, ctypes.c_void_p(frames.data_ptr()), int(T), dt,
                                       ctypes.c_void_p(out.data_ptr()), ms))
@@ -122,8 +130,10 @@ class DPRA:
         with sync=False the call does not wait for the decision (it is taken
         on the device) and returns None (see learned_stats)."""
         dt = self._check_frame(frame)
+        hw = (self.height, self.width)
         if dphi is None:
-            dphi = torch.empty((self.height, self.width), dtype=torch.float32, device=frame.device)
+            dphi = torch.empty(hw, dtype=torch.float32, device=frame.device)
+        check_out(dphi, hw, torch.float32, "dphi")
         res = ctypes.c_double()
         fb = ctypes.c_int()
         check(lib().glsu_dpra_frame_learned(
@@ -146,12 +156,16 @@ class DPRA:
         check(lib().glsu_dpra_copy_net(self._h, ctypes.c_void_p(z.data_ptr())))
         return z
 
-    def get_state(self):
+    def get_state(self, dphiref=True):
+        """(phiref, dphiref) as host float32 arrays; dphiref=False skips the
+        second (returns None in its place)."""
         n = self.height * self.width
         phr = np.empty(n, np.float32)
-        dpr = np.empty(n, np.float32)
-        check(lib().glsu_dpra_get_state(self._h, phr.ctypes.data, dpr.ctypes.data))
-        return phr.reshape(self.height, self.width), dpr.reshape(self.height, self.width)
+        dpr = np.empty(n, np.float32) if dphiref else None
+        check(lib().glsu_dpra_get_state(self._h, phr.ctypes.data,
+                                        dpr.ctypes.data if dphiref else None))
+        return (phr.reshape(self.height, self.width),
+                dpr.reshape(self.height, self.width) if dphiref else None)
 
     def set_state(self, phiref=None, dphiref=None):
         a = None if phiref is None else np.ascontiguousarray(phiref, np.float32)

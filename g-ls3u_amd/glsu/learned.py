@@ -15,7 +15,7 @@ import os
 import numpy as np
 import torch
 
-from ._lib import check, lib
+from ._lib import check, check_out, lib
 from .params import wff_params
 from . import synth
 
@@ -116,6 +116,7 @@ class UNet:
         self._c64(x, "x")
         if out is None:
             out = torch.empty_like(x)
+        check_out(out, x.shape, torch.complex64)
         n, H, W = x.shape
         check(lib().glsu_unet_forward(self._h, ctypes.c_void_p(x.data_ptr()),
                                       ctypes.c_void_p(out.data_ptr()), n, H, W))

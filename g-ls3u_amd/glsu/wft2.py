@@ -10,7 +10,7 @@ import ctypes
 import torch
 
 from . import _lib
-from ._lib import WFF, WFR, Tuning, WfrResults, check, lib, tuning_arg, tuning_dict
+from ._lib import WFF, WFR, Tuning, WfrResults, check, check_out, lib, tuning_arg, tuning_dict
 from .params import wff_params
 
 
@@ -105,6 +105,7 @@ class WFT2:
             raise ValueError(f"f has shape {tuple(f.shape)}, expected {(self.height, self.width)}")
         if out is None:
             out = torch.empty_like(f)
+        check_out(out, (self.height, self.width), torch.complex64)
         ms = ctypes.c_double()
         check(lib().glsu_wft2_compute(self._h, ctypes.c_void_p(f.data_ptr()),
                                       ctypes.c_void_p(out.data_ptr()),

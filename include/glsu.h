@@ -50,13 +50,17 @@ typedef struct glsu_wff_params {
 } glsu_wff_params;
 
 /* Launch/plan choices picked by the autotuner (g-ls3u_amd/glsu/autotune.py).
- * Any field < 0 keeps the library default.  planx / plany: FFT plan ids
- * (glsu_fft_plans) along rows and columns; cols_waves: wavefronts per
- * workgroup of the column kernel (1, 2 or 4); rows_waves: wavefronts per row
- * of the row kernels (1..4).  Results do not depend on these choices beyond
- * FP32 rounding.  No reference counterpart. */
+ * Any field < 0 keeps the library default (ls_tile: also 0).  planx / plany:
+ * FFT plan ids (glsu_fft_plans) along rows and columns; cols_waves:
+ * wavefronts per workgroup of the column kernel (1, 2 or 4); rows_waves:
+ * wavefronts per row of the row kernels (1..4); ls_tile: pixel tile of the
+ * DPRA least-squares kernel, 1 = 64x8 (default), 2 = 64x4, 3 = 32x8,
+ * 4 = 128x4, 5 = 64x16, 6 = 32x16 (batches always use 64x8).  Results do not
+ * depend on the FFT plans beyond FP32 rounding and not at all on the launch
+ * shapes.  No reference counterpart (the reference fixes its DPRA blocks,
+ * DPRA/dpra_cudaf.cu:121-122). */
 typedef struct glsu_tuning {
-  int planx, plany, cols_waves, rows_waves;
+  int planx, plany, cols_waves, rows_waves, ls_tile;
 } glsu_tuning;
 
 typedef struct glsu_wft2 glsu_wft2;

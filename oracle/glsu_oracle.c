@@ -466,18 +466,103 @@ static void spectrum_1d(const fftplan *p, const double *g1, int s, double wt,
     fft_exec(p, out, work, -1);
 }
 
-/* WFF2, WFT/WFT2_CPU.cpp:190-390 (OpenMP over wy, thread-private buffers,
- * serial merge, final scale wxi*wyi/(4*pi^2)). */
-int orc_wft2_compute(orc_wft2 *w, const double *f, double *out) {
+/* One (wy, wx) pair of WFF2 into the thread's accumulator acc,
+ * WFT/WFT2_CPU.cpp:237-353: Fg, sf = ifft2(Ff .* Fg), crop + threshold,
+ * fft2, .* Fg, ifft2, accumulate the crop.  Gy holds the pair's wy spectrum
+ * (separable mode). */
+static void wff2_pair(orc_wft2 *w, int tid, int y, int x, const double *Gy, double *acc) {
     const int W = w->W, H = w->H, PW = w->PW, PH = w->PH, sx = w->sx, sy = w->sy;
+    const size_t np = (size_t)PH * PW;
+    const double inv_np = 1.0 / (double)(PH * PW);
+    const double thr = w->thr;
+    double *gw = w->gwave + 2 * np * tid;
+    double *Fg = w->Fg + 2 * np * tid;
+    double *Sf = w->Sf + 2 * np * tid;
+    double *Gx = w->Gx + 2 * (size_t)PW * tid;
+    fft2ctx *ctx = &w->ctx[tid];
+    double wyt = w->wyl + (double)y * w->wyi;
+    double wxt = w->wxl + (double)x * w->wxi;
+    if (w->flags & ORC_FG_SEPARABLE) {
+        /* Fg(ky,kx) = Gy(ky) Gx(kx): separable form of FFT2(gwave) */
+        spectrum_1d(&w->px[tid], w->gxn, sx, wxt, Gx, ctx->work);
+        for (int i = 0; i < PH; i++)
+            for (int j = 0; j < PW; j++) {
+                size_t id = (size_t)i * PW + j;
+                double ar = Gy[2 * i], ai = Gy[2 * i + 1];
+                double br = Gx[2 * j], bi = Gx[2 * j + 1];
+                Fg[2 * id] = ar * br - ai * bi;
+                Fg[2 * id + 1] = ar * bi + ai * br;
+            }
+    } else {
+        /* gwave = g .* exp(j(wxt*xx + wyt*yy)), :247-266; Fg = fft2(gwave), :268 */
+        memset(gw, 0, sizeof(double) * 2 * np);
+        for (int i = 0; i < 2 * sy + 1; i++)
+            for (int j = 0; j < 2 * sx + 1; j++) {
+                size_t id = (size_t)i * PW + j;
+                double yy = i - sy, xx = j - sx;
+                double c = cos(wxt * xx + wyt * yy), s = sin(wxt * xx + wyt * yy);
+                double g = w->gpad[2 * id];
+                gw[2 * id] = g * c;
+                gw[2 * id + 1] = g * s;
+            }
+        memcpy(Fg, gw, sizeof(double) * 2 * np);
+        fft2_exec(ctx, Fg, -1);
+    }
+    /* sf = ifft2(Ff .* Fg), :271-285 */
+    for (size_t id = 0; id < np; id++) {
+        double ar = w->Ff[2 * id], ai = w->Ff[2 * id + 1];
+        double br = Fg[2 * id], bi = Fg[2 * id + 1];
+        Sf[2 * id] = (ar * br - ai * bi) * inv_np;
+        Sf[2 * id + 1] = (ar * bi + ai * br) * inv_np;
+    }
+    fft2_exec(ctx, Sf, +1);
+    /* crop (+sy,+sx) and hard threshold |sf| >= thr, :289-319 */
+    for (int i = 0; i < PH; i++)
+        for (int j = 0; j < PW; j++) {
+            size_t idp = (size_t)i * PW + j;
+            if (i < H && j < W) {
+                size_t idt = (size_t)(i + sy) * PW + j + sx;
+                double re = Sf[2 * idt], im = Sf[2 * idt + 1];
+                double a = sqrt(re * re + im * im);
+                if (a >= thr) { Sf[2 * idp] = re; Sf[2 * idp + 1] = im; }
+                else { Sf[2 * idp] = 0; Sf[2 * idp + 1] = 0; }
+                Sf[2 * idt] = 0; Sf[2 * idt + 1] = 0;
+            } else {
+                Sf[2 * idp] = 0; Sf[2 * idp + 1] = 0;
+            }
+        }
+    /* fft2, .* Fg / Np, ifft2, :322-339 */
+    fft2_exec(ctx, Sf, -1);
+    for (size_t id = 0; id < np; id++) {
+        double ar = Sf[2 * id], ai = Sf[2 * id + 1];
+        double br = Fg[2 * id], bi = Fg[2 * id + 1];
+        Sf[2 * id] = (ar * br - ai * bi) * inv_np;
+        Sf[2 * id + 1] = (ar * bi + ai * br) * inv_np;
+    }
+    fft2_exec(ctx, Sf, +1);
+    /* accumulate the (+sy,+sx) crop, :341-352 */
+    for (int i = 0; i < H; i++)
+        for (int j = 0; j < W; j++) {
+            size_t idf = (size_t)i * W + j;
+            size_t ids = (size_t)(i + sy) * PW + (j + sx);
+            acc[2 * idf] += Sf[2 * ids];
+            acc[2 * idf + 1] += Sf[2 * ids + 1];
+        }
+}
+
+/* WFF2, WFT/WFT2_CPU.cpp:190-390 (OpenMP over wy, thread-private buffers,
+ * serial merge, final scale wxi*wyi/(4*pi^2)).  With ORC_PAIR_PARALLEL the
+ * threads take (wy, wx) pairs instead of wy rows: the same arithmetic per
+ * pair, more parallelism (the sum of the per-thread accumulators is in a
+ * different order, a difference at FP64 rounding level). */
+int orc_wft2_compute(orc_wft2 *w, const double *f, double *out) {
+    const int W = w->W, H = w->H, PW = w->PW, PH = w->PH, sy = w->sy;
     const size_t np = (size_t)PH * PW, ni = (size_t)W * H;
     wff2_set_threshold_and_feed(w, f);
     memcpy(w->Ff, w->fpad, sizeof(double) * 2 * np);
     fft2_exec(&w->ctx[0], w->Ff, -1);
     memset(w->filt, 0, sizeof(double) * 2 * ni * w->nthreads);
     const int iwx = w->nwx, iwy = w->nwy;
-    const double inv_np = 1.0 / (double)(PH * PW);
-    const double thr = w->thr;
 #pragma omp parallel num_threads(w->nthreads)
     {
         int tid = 0, nth = 1;
@@ -485,85 +570,23 @@ int orc_wft2_compute(orc_wft2 *w, const double *f, double *out) {
         tid = omp_get_thread_num();
         nth = omp_get_num_threads();
 #endif
-        double *gw = w->gwave + 2 * np * tid;
-        double *Fg = w->Fg + 2 * np * tid;
-        double *Sf = w->Sf + 2 * np * tid;
         double *acc = w->filt + 2 * ni * tid;
-        double *Gx = w->Gx + 2 * (size_t)PW * tid;
         double *Gy = w->Gy + 2 * (size_t)PH * tid;
         fft2ctx *ctx = &w->ctx[tid];
-        for (int y = tid; y < iwy; y += nth) {
-            double wyt = w->wyl + (double)y * w->wyi;
-            if (w->flags & ORC_FG_SEPARABLE)
-                spectrum_1d(&w->py[tid], w->gyn, sy, wyt, Gy, ctx->work);
-            for (int x = 0; x < iwx; x++) {
-                double wxt = w->wxl + (double)x * w->wxi;
-                if (w->flags & ORC_FG_SEPARABLE) {
-                    /* Fg(ky,kx) = Gy(ky) Gx(kx): separable form of FFT2(gwave) */
-                    spectrum_1d(&w->px[tid], w->gxn, sx, wxt, Gx, ctx->work);
-                    for (int i = 0; i < PH; i++)
-                        for (int j = 0; j < PW; j++) {
-                            size_t id = (size_t)i * PW + j;
-                            double ar = Gy[2 * i], ai = Gy[2 * i + 1];
-                            double br = Gx[2 * j], bi = Gx[2 * j + 1];
-                            Fg[2 * id] = ar * br - ai * bi;
-                            Fg[2 * id + 1] = ar * bi + ai * br;
-                        }
-                } else {
-                    /* gwave = g .* exp(j(wxt*xx + wyt*yy)), :247-266; Fg = fft2(gwave), :268 */
-                    memset(gw, 0, sizeof(double) * 2 * np);
-                    for (int i = 0; i < 2 * sy + 1; i++)
-                        for (int j = 0; j < 2 * sx + 1; j++) {
-                            size_t id = (size_t)i * PW + j;
-                            double yy = i - sy, xx = j - sx;
-                            double c = cos(wxt * xx + wyt * yy), s = sin(wxt * xx + wyt * yy);
-                            double g = w->gpad[2 * id];
-                            gw[2 * id] = g * c;
-                            gw[2 * id + 1] = g * s;
-                        }
-                    memcpy(Fg, gw, sizeof(double) * 2 * np);
-                    fft2_exec(ctx, Fg, -1);
-                }
-                /* sf = ifft2(Ff .* Fg), :271-285 */
-                for (size_t id = 0; id < np; id++) {
-                    double ar = w->Ff[2 * id], ai = w->Ff[2 * id + 1];
-                    double br = Fg[2 * id], bi = Fg[2 * id + 1];
-                    Sf[2 * id] = (ar * br - ai * bi) * inv_np;
-                    Sf[2 * id + 1] = (ar * bi + ai * br) * inv_np;
-                }
-                fft2_exec(ctx, Sf, +1);
-                /* crop (+sy,+sx) and hard threshold |sf| >= thr, :289-319 */
-                for (int i = 0; i < PH; i++)
-                    for (int j = 0; j < PW; j++) {
-                        size_t idp = (size_t)i * PW + j;
-                        if (i < H && j < W) {
-                            size_t idt = (size_t)(i + sy) * PW + j + sx;
-                            double re = Sf[2 * idt], im = Sf[2 * idt + 1];
-                            double a = sqrt(re * re + im * im);
-                            if (a >= thr) { Sf[2 * idp] = re; Sf[2 * idp + 1] = im; }
-                            else { Sf[2 * idp] = 0; Sf[2 * idp + 1] = 0; }
-                            Sf[2 * idt] = 0; Sf[2 * idt + 1] = 0;
-                        } else {
-                            Sf[2 * idp] = 0; Sf[2 * idp + 1] = 0;
-                        }
-                    }
-                /* fft2, .* Fg / Np, ifft2, :322-339 */
-                fft2_exec(ctx, Sf, -1);
-                for (size_t id = 0; id < np; id++) {
-                    double ar = Sf[2 * id], ai = Sf[2 * id + 1];
-                    double br = Fg[2 * id], bi = Fg[2 * id + 1];
-                    Sf[2 * id] = (ar * br - ai * bi) * inv_np;
-                    Sf[2 * id + 1] = (ar * bi + ai * br) * inv_np;
-                }
-                fft2_exec(ctx, Sf, +1);
-                /* accumulate the (+sy,+sx) crop, :341-352 */
-                for (int i = 0; i < H; i++)
-                    for (int j = 0; j < W; j++) {
-                        size_t idf = (size_t)i * W + j;
-                        size_t ids = (size_t)(i + sy) * PW + (j + sx);
-                        acc[2 * idf] += Sf[2 * ids];
-                        acc[2 * idf + 1] += Sf[2 * ids + 1];
-                    }
+        if (w->flags & ORC_PAIR_PARALLEL) {
+            for (int p = tid; p < iwy * iwx; p += nth) {
+                int y = p / iwx, x = p % iwx;
+                if (w->flags & ORC_FG_SEPARABLE)
+                    spectrum_1d(&w->py[tid], w->gyn, sy, w->wyl + (double)y * w->wyi, Gy,
+                                ctx->work);
+                wff2_pair(w, tid, y, x, Gy, acc);
+            }
+        } else {
+            for (int y = tid; y < iwy; y += nth) {
+                if (w->flags & ORC_FG_SEPARABLE)
+                    spectrum_1d(&w->py[tid], w->gyn, sy, w->wyl + (double)y * w->wyi, Gy,
+                                ctx->work);
+                for (int x = 0; x < iwx; x++) wff2_pair(w, tid, y, x, Gy, acc);
             }
         }
     }
@@ -601,7 +624,9 @@ orc_dpra *orc_dpra_create(const double *phi0, int W, int H, int rr,
                           double thr, int nthreads, int flags) {
     if (W < 1 || H < 1 || rr < 1 || nthreads < 1) return NULL;
     orc_wft2 *wft = orc_wft2_create(W, H, sigx, wxl, wxh, wxi, sigy, wyl, wyh, wyi, thr,
-                                    nthreads, ORC_FG_SEPARABLE);
+                                    nthreads,
+                                    ORC_FG_SEPARABLE | (flags & ORC_DPRA_PAIR_PARALLEL
+                                                        ? ORC_PAIR_PARALLEL : 0));
     if (!wft) return NULL;
     orc_dpra *d = (orc_dpra *)calloc(1, sizeof(orc_dpra));
     size_t n = (size_t)W * H;
@@ -728,6 +753,12 @@ void orc_dpra_update_ref(orc_dpra *d) {
     const size_t n = (size_t)d->W * d->H;
     memcpy(d->phiref, d->phicurr, sizeof(double) * n);
     memcpy(d->dphiref, d->dphi_frame, sizeof(double) * n);
+}
+
+void orc_dpra_set_state(orc_dpra *d, const double *phiref, const double *dphiref) {
+    const size_t n = (size_t)d->W * d->H;
+    if (phiref) memcpy(d->phiref, phiref, sizeof(double) * n);
+    if (dphiref) memcpy(d->dphiref, dphiref, sizeof(double) * n);
 }
 
 /* operator(), DPRA/dpra_cpu.cpp:58-89 */

@@ -46,6 +46,10 @@ void orc_gen_frame(int W, int H, int t, uint64_t seed, double amp0, double amp1,
 /* ---- WFT2 (WFF) ------------------------------------------------------------ */
 typedef struct orc_wft2 orc_wft2;
 enum { ORC_FG_SEPARABLE = 1 };   /* Fg as outer product of 1-D DFTs (exact form) */
+/* Threads take (wy, wx) pairs instead of wy rows (the reference's loop,
+ * WFT/WFT2_CPU.cpp:227-237, parallelises over wy only).  Same arithmetic per
+ * pair; used where the oracle checks long sequences, not for the baseline. */
+enum { ORC_PAIR_PARALLEL = 2 };
 
 orc_wft2 *orc_wft2_create(int W, int H,
                           double sigx, double wxl, double wxh, double wxi,
@@ -65,6 +69,7 @@ int    orc_wfr2_compute(orc_wft2 *w, const double *f, double *out);
 /* ---- DPRA (DPRA_CPU) ------------------------------------------------------ */
 typedef struct orc_dpra orc_dpra;
 enum { ORC_NO_FLOAT_CAST = 1 };  /* disable the (float) cast of dpra_cpu.cpp:246 */
+enum { ORC_DPRA_PAIR_PARALLEL = 2 };  /* WFF2 with ORC_PAIR_PARALLEL */
 
 orc_dpra *orc_dpra_create(const double *phi0, int W, int H, int rr,
                           double sigx, double wxl, double wxh, double wxi,
@@ -76,6 +81,10 @@ int  orc_dpra_ls(orc_dpra *d, const double *frame, double *z);
 /* one frame: LS + WFF2 + phase update (dpra_per_frame).  phicurr may be NULL. */
 int  orc_dpra_frame(orc_dpra *d, const double *frame, double *dphi, double *phicurr);
 void orc_dpra_update_ref(orc_dpra *d);
+/* Overwrite the reference phase and the cumulative phase change (H*W doubles
+ * each; either may be NULL).  The reference has no such call: its state is
+ * phi0 and zero at construction (DPRA/dpra_cpu.cpp:19, :39-43). */
+void orc_dpra_set_state(orc_dpra *d, const double *phiref, const double *dphiref);
 /* operator(): frames T*H*W, dphi_all T*H*W, with i % rr == 0 reference updates */
 int  orc_dpra_run(orc_dpra *d, const double *frames, int T, double *dphi_all);
 /* per-pixel solve used by DPRA (exposed for pin tests): A col-major 3x3 (only

@@ -57,6 +57,7 @@ def lib():
             "orc_dpra_ls": (I, [V, _dp, _dp]),
             "orc_dpra_frame": (I, [V, _dp, _dp, _dp]),
             "orc_dpra_update_ref": (None, [V]),
+            "orc_dpra_set_state": (None, [V, _dp, _dp]),
             "orc_dpra_run": (I, [V, _dp, I, _dp]),
             "orc_solve3_sym": (I, [_dp, _dp]),
             "orc_dpra_thr": (D, [V]),
@@ -151,19 +152,21 @@ def gen_sequence(W, H, T, seed, amp0=3.0, amp1=0.1, carrier=0.05, noise=10.0,
 
 
 class WFF2:
-    """FP64 WFF2 (WFT2_cpu in WFF mode)."""
+    """FP64 WFF2 (WFT2_cpu in WFF mode).  pair_parallel: threads take (wy, wx)
+    pairs rather than wy rows (same arithmetic per pair)."""
 
     def __init__(self, W, H, sigx=10.0, wxl=-0.2, wxh=0.2, wxi=0.1,
                  sigy=None, wyl=None, wyh=None, wyi=None, thr=-1.0,
-                 nthreads=None, separable=True):
+                 nthreads=None, separable=True, pair_parallel=False):
         sigy = sigx if sigy is None else sigy
         wyl = wxl if wyl is None else wyl
         wyh = wxh if wyh is None else wyh
         wyi = wxi if wyi is None else wyi
         nthreads = nthreads or min(os.cpu_count() or 1, 16)
         self.W, self.H = W, H
+        flags = (1 if separable else 0) | (2 if pair_parallel else 0)
         self._h = lib().orc_wft2_create(W, H, sigx, wxl, wxh, wxi, sigy, wyl, wyh,
-                                        wyi, thr, nthreads, 1 if separable else 0)
+                                        wyi, thr, nthreads, flags)
         if not self._h:
             raise ValueError("FFT padding is out of range [4096]")
 
@@ -221,11 +224,13 @@ class WFR2(WFF2):
 
 
 class DPRA:
-    """FP64 DPRA_CPU chain (LS + WFF2 + update)."""
+    """FP64 DPRA_CPU chain (LS + WFF2 + update).  pair_parallel: WFF2 threads
+    take (wy, wx) pairs rather than wy rows (same arithmetic; for checking
+    long sequences -- the CPU baseline keeps the reference's wy loop)."""
 
     def __init__(self, phi0, rr=1, sigx=10.0, wxl=-0.2, wxh=0.2, wxi=0.1,
                  sigy=None, wyl=None, wyh=None, wyi=None, thr=-1.0,
-                 nthreads=None, float_cast=True):
+                 nthreads=None, float_cast=True, pair_parallel=False):
         phi0 = _f64(phi0)
         H, W = phi0.shape
         sigy = sigx if sigy is None else sigy
@@ -236,7 +241,7 @@ class DPRA:
         self.W, self.H = W, H
         self._h = lib().orc_dpra_create(_ptr(phi0), W, H, rr, sigx, wxl, wxh, wxi,
                                         sigy, wyl, wyh, wyi, thr, nthreads,
-                                        0 if float_cast else 1)
+                                        (0 if float_cast else 1) | (2 if pair_parallel else 0))
         if not self._h:
             raise ValueError("invalid DPRA parameters or FFT padding out of range")
 
@@ -264,6 +269,15 @@ class DPRA:
 
     def update_ref_phi(self):
         lib().orc_dpra_update_ref(self._h)
+
+    def set_state(self, phiref=None, dphiref=None):
+        """Overwrite the reference phase and/or the cumulative phase change."""
+        a = None if phiref is None else _f64(phiref)
+        b = None if dphiref is None else _f64(dphiref)
+        for v in (a, b):
+            assert v is None or v.shape == (self.H, self.W)
+        lib().orc_dpra_set_state(self._h, _ptr(a) if a is not None else None,
+                                 _ptr(b) if b is not None else None)
 
     def run(self, frames):
         fr = _f64(frames)

@@ -91,6 +91,7 @@ def test_tuning_errors():
     assert create(cols_waves=3) == _lib.ERR_ARG
     assert create(rows_waves=5) == _lib.ERR_ARG
     assert create(rows_waves=0) == _lib.ERR_ARG
+    assert create(ls_tile=7) == _lib.ERR_ARG            # unknown least-squares tile shape
     plans = {q["id"]: q for q in glsu.fft_plans()}
     narrow = [i for i, q in plans.items() if q["T"] != 64 and q["N"] >= 200 + 30]
     short = [i for i, q in plans.items() if q["N"] < 150]
@@ -129,3 +130,33 @@ def test_cpp_wrapper_compiles():
         r = subprocess.run(["/opt/rocm/bin/hipcc", "-std=c++17", "-fsyntax-only", src],
                            capture_output=True, text=True)
         assert r.returncode == 0, name + ": " + r.stderr[-2000:]
+
+
+def test_output_tensors_are_validated_before_any_launch():
+    """Caller-supplied outputs go to the C ABI as raw pointers; a tensor of the
+    wrong shape, dtype, device or layout must be refused in Python first."""
+    torch = pytest.importorskip("torch")
+    chk = _lib.check_out
+    with pytest.raises(ValueError, match="CUDA"):
+        chk(torch.empty((3, 4, 5), dtype=torch.float32), (3, 4, 5), torch.float32)
+    with pytest.raises(ValueError, match="CUDA"):
+        chk(None, (3, 4, 5), torch.float32)
+
+    class CudaLike:
+        """Metadata of a CUDA tensor (the checks read nothing else)."""
+        is_cuda = True
+
+        def __init__(self, t):
+            self.t, self.dtype, self.shape = t, t.dtype, t.shape
+
+        def is_contiguous(self):
+            return self.t.is_contiguous()
+
+    good = CudaLike(torch.empty((3, 4, 5), dtype=torch.float32))
+    assert chk(good, (3, 4, 5), torch.float32) is good
+    with pytest.raises(ValueError, match="shape"):
+        chk(good, (3, 4, 6), torch.float32)
+    with pytest.raises(ValueError, match="float64"):
+        chk(CudaLike(torch.empty((3, 4, 5), dtype=torch.float64)), (3, 4, 5), torch.float32)
+    with pytest.raises(ValueError, match="contiguous"):
+        chk(CudaLike(torch.empty((5, 4, 3)).transpose(0, 2)), (3, 4, 5), torch.float32)

@@ -60,6 +60,7 @@ def test_tune_picks_valid_cached_choice(oracle, tmp_path):
     assert plans[res["planx"]]["T"] == 64 and plans[res["planx"]]["N"] >= W + 30
     assert plans[res["plany"]]["N"] >= H + 30
     assert res["cols_waves"] in (1, 2, 4) and 1 <= res["rows_waves"] <= 4
+    assert res["ls_tile"] in glsu._lib.LS_TILES
     assert len(res["ms"]) == 4 and all(m > 0 for m in res["ms"])
     # second call: served from the cache
     again = autotune.tune(W, H, cache=cache)
@@ -79,3 +80,19 @@ def test_tune_picks_valid_cached_choice(oracle, tmp_path):
         say(f"   frame {i}: tuned vs oracle rms={rms:.2e} max={mx:.2e}; vs default rms={rms2:.2e}")
         assert rms < 1e-3
         assert rms2 < 1e-4
+
+
+def test_every_ls_tile_gives_bitwise_identical_chains(oracle):
+    """The least-squares tile shape changes only how pixels are grouped into
+    workgroups: every shape gives the default's chain output bit for bit
+    (odd size, so partial tiles at both edges)."""
+    W, H, T = 333, 211, 3
+    frames, phi0 = oracle.gen_sequence(W, H, T, 23, dtype=np.float32)
+    fr = torch.from_numpy(frames).cuda()
+    base = glsu.DPRA(phi0)(fr).cpu().numpy()
+    for tile, shape in glsu._lib.LS_TILES.items():
+        d = glsu.DPRA(phi0, tuning=dict(ls_tile=tile))
+        assert d.tuning()["ls_tile"] == tile
+        out = d(fr).cpu().numpy()
+        say(f"[ls tile {tile} = {shape}] bitwise equal to the default: {np.array_equal(out, base)}")
+        assert np.array_equal(out, base), np.abs(out - base).max()

@@ -1,0 +1,139 @@
+"""Whole-sequence parity of the DPRA chain against the FP64 oracle.
+
+The chain is recursive: frame t is demodulated against the reference phase
+that frames 0..t-1 produced (DPRA/dpra_cpu.cpp:58-89, :244-258).  Two views:
+
+* one-step parity at every frame (the C2 sequence, 1024^2, 200 frames): the
+  GPU's own reference phase phi_r(t) is read back before frame t, and the
+  oracle computes frame t's phase increment from that same phi_r(t).  Every
+  output pixel of one step depends only on its input within +-(2s + 1) (two
+  convolutions of reach s with a pointwise threshold between them, plus the
+  3x3 least squares), so the oracle run on a (64 + 2(2s+1))^2 window around a
+  64x64 tile reproduces the full-image step on the tile exactly.  This checks
+  the arithmetic of every frame independently of how the recursion carries
+  earlier differences forward.
+* the chained comparison (the C1 sequence, 256^2, 20 frames) against the
+  oracle chain run from the same float32 phi0 the GPU receives, and against
+  the oracle chain from the exact phi0.
+"""
+import numpy as np
+import pytest
+
+from _util import say, wrapped_err
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+import glsu  # noqa: E402
+
+
+def _wrap(a):
+    return np.angle(np.exp(1j * a))
+
+
+def _tiles(W, H, t, n=64):
+    """Three 64x64 tile origins for frame t: one corner (rotating through the
+    four) and two drawn from a per-frame seed."""
+    corners = [(0, 0), (W - n, 0), (0, H - n), (W - n, H - n)]
+    rng = np.random.default_rng(1000 + t)
+    return [corners[t % 4]] + [(int(rng.integers(0, W - n + 1)), int(rng.integers(0, H - n + 1)))
+                               for _ in range(2)]
+
+
+def test_c2_one_step_tile_parity_every_frame(oracle):
+    """C2 (1024^2, 200 frames, sigma 10, 25 pairs, noise 10, carrier 0.05):
+    at every frame, the GPU's increment wrap(phi_c - phi_r) on three tiles
+    against the FP64 oracle's one step from the GPU's phi_r on the tiles'
+    windows.  Bar: <= 1e-3 rad RMS on every frame."""
+    W = H = 1024
+    T = 200
+    n, s = 64, 30
+    m = 2 * s + 1
+    seed = 20240607
+    say(f"[c2 one-step] {W}x{H} T={T}")
+    frames = glsu.gen_frames(W, H, T, seed, 3.0, 0.1, 0.05, 10.0)
+    host = frames.cpu().numpy()
+    phi0 = glsu.synth.phase(W, H, 3.0, 0.05)
+    d = glsu.DPRA(phi0)
+    pc = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    dphi = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    orcs = {}
+    rms_all, max_all = [], []
+    for t in range(T):
+        phr, _ = d.get_state(dphiref=False)
+        d.dpra_per_frame(frames[t], dphi=dphi, phicurr=pc)
+        d.update_ref_phi()
+        inc = _wrap(pc.cpu().numpy().astype(np.float64) - phr.astype(np.float64))
+        errs = []
+        for (x0, y0) in _tiles(W, H, t, n):
+            wx0, wy0 = max(x0 - m, 0), max(y0 - m, 0)
+            wx1, wy1 = min(x0 + n + m, W), min(y0 + n + m, H)
+            key = (wy1 - wy0, wx1 - wx0)
+            if key not in orcs:
+                orcs[key] = oracle.DPRA(np.zeros(key), pair_parallel=True)
+            od = orcs[key]
+            od.set_state(phr[wy0:wy1, wx0:wx1], np.zeros(key))
+            ref, _ = od.dpra_per_frame(host[t, wy0:wy1, wx0:wx1].astype(np.float64))
+            r = ref[y0 - wy0:y0 - wy0 + n, x0 - wx0:x0 - wx0 + n]
+            errs.append(_wrap(inc[y0:y0 + n, x0:x0 + n] - r).ravel())
+        e = np.concatenate(errs)
+        rms, mx = float(np.sqrt(np.mean(e * e))), float(np.abs(e).max())
+        rms_all.append(rms)
+        max_all.append(mx)
+        say(f"   frame {t}: tiles {_tiles(W, H, t, n)} rms={rms:.3e} max={mx:.3e}")
+    say(f"   over {T} frames: rms max {max(rms_all):.3e}, mean {np.mean(rms_all):.3e}; "
+        f"max |e| {max(max_all):.3e}")
+    assert max(rms_all) <= 1e-3
+
+
+def test_c1_chain_against_float32_phi0_oracle(oracle):
+    """C1 (256^2, 20 frames, amplitude 3 + 0.1 t, carrier 0.05, noise 10).
+    The GPU stores phi0 in float32; the fair comparator is the FP64 oracle
+    chain run from that same float32 phi0 (ref32).  Printed per frame: GPU vs
+    ref32, GPU vs the oracle from the exact phi0 (ref), and ref32 vs ref (how
+    far the oracle itself moves for a float32 rounding of phi0)."""
+    W = H = 256
+    T = 20
+    say(f"[chain-c1] {W}x{H} T={T}")
+    frames, phi0 = oracle.gen_sequence(W, H, T, 7, dtype=np.float32)
+    f64 = frames.astype(np.float64)
+    ref = oracle.DPRA(phi0).run(f64)
+    ref32 = oracle.DPRA(phi0.astype(np.float32).astype(np.float64)).run(f64)
+    out = glsu.DPRA(phi0)(torch.from_numpy(frames).cuda()).cpu().numpy()
+    g32, g64, fl = [], [], []
+    for t in range(T):
+        g32.append(wrapped_err(out[t], ref32[t]))
+        g64.append(wrapped_err(out[t], ref[t]))
+        fl.append(wrapped_err(ref32[t], ref[t]))
+        say(f"   frame {t}: gpu-ref32 rms={g32[-1][0]:.3e} max={g32[-1][1]:.3e} | "
+            f"gpu-ref rms={g64[-1][0]:.3e} max={g64[-1][1]:.3e} | "
+            f"ref32-ref rms={fl[-1][0]:.3e} max={fl[-1][1]:.3e}")
+    assert max(r for r, _ in g32) <= 1e-3
+
+
+def test_c1_one_step_parity_every_frame(oracle):
+    """C1 (256^2, 20 frames): at every frame the GPU's increment against the
+    FP64 oracle's one step from the GPU's own phi_r(t) on the full image.
+    Bar: <= 1e-3 rad RMS on every frame."""
+    W = H = 256
+    T = 20
+    frames, phi0 = oracle.gen_sequence(W, H, T, 7, dtype=np.float32)
+    fr = torch.from_numpy(frames).cuda()
+    d = glsu.DPRA(phi0)
+    od = oracle.DPRA(np.zeros((H, W)), pair_parallel=True)
+    pc = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    worst = 0.0
+    for t in range(T):
+        phr, _ = d.get_state(dphiref=False)
+        d.dpra_per_frame(fr[t], phicurr=pc)
+        d.update_ref_phi()
+        inc = _wrap(pc.cpu().numpy().astype(np.float64) - phr.astype(np.float64))
+        od.set_state(phr, np.zeros((H, W)))
+        ref, _ = od.dpra_per_frame(frames[t].astype(np.float64))
+        rms, mx = wrapped_err(inc, ref)
+        worst = max(worst, rms)
+        say(f"[c1 one-step] frame {t}: rms={rms:.3e} max={mx:.3e}")
+    assert worst <= 1e-3

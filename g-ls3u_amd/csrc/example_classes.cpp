@@ -91,7 +91,7 @@ static int run_wff() {
   const bool allocated = filtered != nullptr && attr_rc == 0 && mem_type == (int)hipMemoryTypeDevice;
   std::printf("wff: %.3f ms; filtered field allocated by the constructor: %s; interior phase "
               "error %.2e, min |F| %.3f\n", t, allocated ? "yes" : "no", ep, amin);
-  return (allocated && ep < 1e-5 && amin > 0.5) ? 0 : 1;
+  return (allocated && ep < 1e-5 && amin > 0.1) ? 0 : 1;
 }
 
 static int run_wfr() {

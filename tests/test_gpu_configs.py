@@ -202,8 +202,11 @@ def test_threshold_computed_on_first_call_and_kept(oracle):
     assert abs(fresh.info()["thr"] - expect2) <= 1e-5 * expect2
     for g, r in ((g1, r1), (g2, r2)):
         rel = np.abs(g - r).max() / np.abs(r).max()
-        rms, _ = wrapped_err(np.angle(g), np.angle(r))
-        say(f"   vs oracle: rel {rel:.2e}, phase rms {rms:.2e}")
+        # phase where the filtered field is not vanishingly small (the phase of
+        # a near-zero value carries no information)
+        big = np.abs(r) > 1e-2 * np.abs(r).max()
+        rms, _ = wrapped_err(np.angle(g[big]), np.angle(r[big]))
+        say(f"   vs oracle: rel {rel:.2e}, phase rms {rms:.2e} on {big.mean():.3f} of the pixels")
         assert rel < 1e-4 and rms < 1e-4
     # the kept threshold matters: a fresh handle's second-field output differs
     assert np.abs(g2_fresh - g2).max() > 1e-3 * np.abs(g2).max()

@@ -78,28 +78,6 @@ def test_dpra_chain_matches_oracle(oracle, W, H, T, amp0, amp1, carrier):
         assert rms < 1e-3
 
 
-def test_dpra_chain_c1_vs_oracle_floor(oracle):
-    """The C1 sequence (256^2, 20 frames, amplitude 3 + 0.1 t).  At this size
-    the peaks gradient cancels the carrier on a few percent of pixels, where
-    the 3x3 system is near-singular and the chain amplifies any perturbation
-    about 3x per frame.  The oracle's own floor is measured by re-running it
-    with phi0 rounded to float32 (the GPU's storage precision); the GPU must
-    stay within 1e-3 rad RMS or within 3x that floor, whichever is larger."""
-    W = H = 256
-    T = 20
-    say(f"[chain-c1] {W}x{H} T={T}")
-    frames, phi0 = _seq(oracle, W, H, T, seed=7)
-    f64 = frames.astype(np.float64)
-    ref = oracle.DPRA(phi0).run(f64)
-    ref32 = oracle.DPRA(phi0.astype(np.float32).astype(np.float64)).run(f64)
-    out = glsu.DPRA(phi0)(torch.from_numpy(frames).cuda()).cpu().numpy()
-    for t in range(T):
-        rms, mx = wrapped_err(out[t], ref[t])
-        floor, _ = wrapped_err(ref32[t], ref[t])
-        say(f"   frame {t}: gpu rms={rms:.3e} max={mx:.3e} | oracle floor rms={floor:.3e}")
-        assert rms <= max(1e-3, 3 * floor)
-
-
 def test_graph_equals_eager(oracle):
     """hipGraph replays of the frame chain equal the eager chain bit for bit;
     the same handle replays its captured graphs (calls 3 and 4), with eager

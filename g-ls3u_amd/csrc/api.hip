@@ -794,8 +794,43 @@ extern "C" int glsu_dpra_run_graph(glsu_dpra* h, const void* frames, int nframes
 // Per-kernel device times over a frame loop (events between the kernels of
 // each frame).  ms[0..3] receive the summed times of the least-squares, row
 // forward, column and row inverse + update kernels.  Diagnostics only.
+static int dpra_profile_impl(glsu_dpra* h, const void* frames, int nframes, int dtype,
+                             float* dphi_all, double* ms);
+
 extern "C" int glsu_dpra_profile(glsu_dpra* h, const void* frames, int nframes, int dtype,
                                  float* dphi_all, double* ms) {
+  return dpra_profile_impl(h, frames, nframes, dtype, dphi_all, ms);
+}
+
+// The same, counting the column transforms skipped because a column's
+// thresholded signal was zero on every image row (a device counter the
+// column kernel increments only in this call).
+extern "C" int glsu_dpra_profile_skips(glsu_dpra* h, const void* frames, int nframes, int dtype,
+                                       float* dphi_all, double* ms, unsigned long long* skipped,
+                                       unsigned long long* transforms) {
+  if (!h || !skipped || !transforms) return fail(GLSU_ERR_ARG, "invalid argument");
+  unsigned int* cnt = nullptr;
+  HIP_TRY(hipMalloc((void**)&cnt, sizeof(unsigned int)));
+  int rc = GLSU_OK;
+  if (hipMemsetAsync(cnt, 0, sizeof(unsigned int), h->s) != hipSuccess) {
+    rc = fail(GLSU_ERR_DEVICE, "counter reset failed");
+  } else {
+    h->p.d_nskip = cnt;
+    rc = dpra_profile_impl(h, frames, nframes, dtype, dphi_all, ms);
+    h->p.d_nskip = nullptr;
+  }
+  unsigned int v = 0;
+  if (!rc && hipMemcpy(&v, cnt, sizeof v, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(GLSU_ERR_DEVICE, "counter read failed");
+  hipFree(cnt);
+  if (rc) return rc;
+  *skipped = v;
+  *transforms = (unsigned long long)nframes * h->p.nwx * h->W * h->p.nwy;
+  return GLSU_OK;
+}
+
+static int dpra_profile_impl(glsu_dpra* h, const void* frames, int nframes, int dtype,
+                             float* dphi_all, double* ms) {
   if (!h || !frames || !ms || nframes < 0) return fail(GLSU_ERR_ARG, "invalid argument");
   const int es = elem_size(dtype);
   if (!es) return fail(GLSU_ERR_ARG, "unknown frame dtype %d", dtype);

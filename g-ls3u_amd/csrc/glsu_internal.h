@@ -42,6 +42,7 @@ struct WffPlan {
   float2* d_D = nullptr;    // [nwx] tiled planes: column-filtered field
   float* d_thr = nullptr;   // threshold (device scalar)
   float* d_red = nullptr;   // reduction scratch
+  unsigned int* d_nskip = nullptr;  // diagnostics: skipped column transforms (null = off)
   size_t plane_elems = 0;   // elements per tiled plane
 };
 

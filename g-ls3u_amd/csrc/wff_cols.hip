@@ -15,17 +15,17 @@ hipError_t launch_cols(const WffPlan& p, hipStream_t s, const int* skip) {
       case 1:                                                                                   \
         hipLaunchKernelGGL((wff_cols_kernel<P, 1>), dim3(grid), dim3(64), smem, s, p.d_A,      \
                            p.d_D, p.d_Hy, p.d_twy, p.d_thr, p.W, p.H, p.nwx, p.nwy,             \
-                           p.plane_elems, skip);                                                \
+                           p.plane_elems, skip, p.d_nskip);                                     \
         break;                                                                                  \
       case 2:                                                                                   \
         hipLaunchKernelGGL((wff_cols_kernel<P, 2>), dim3(grid), dim3(128), smem, s, p.d_A,     \
                            p.d_D, p.d_Hy, p.d_twy, p.d_thr, p.W, p.H, p.nwx, p.nwy,             \
-                           p.plane_elems, skip);                                                \
+                           p.plane_elems, skip, p.d_nskip);                                     \
         break;                                                                                  \
       default:                                                                                  \
         hipLaunchKernelGGL((wff_cols_kernel<P, 4>), dim3(grid), dim3(256), smem, s, p.d_A,     \
                            p.d_D, p.d_Hy, p.d_twy, p.d_thr, p.W, p.H, p.nwx, p.nwy,             \
-                           p.plane_elems, skip);                                                \
+                           p.plane_elems, skip, p.d_nskip);                                     \
         break;                                                                                  \
     }                                                                                           \
   }

@@ -152,7 +152,8 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
                                           const float* __restrict__ Hy,
                                           const float2* __restrict__ tw_g, float thr, int W,
                                           int H, int nwx, int nwy, size_t plane, int item,
-                                          int lane, const float2* ttab, float2* buf) {
+                                          int lane, const float2* ttab, float2* buf,
+                                          unsigned int* nskip = nullptr) {
   constexpr int T = P::T, E = P::E, N = P::N;
   using Ctx = FftLane<P, true>;
   const int k = item / W, x = item % W;
@@ -199,7 +200,10 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
     // a column whose thresholded signal is zero everywhere adds exactly
     // nothing: skip its transform when no slot of the wavefront has a
     // surviving sample (wave-uniform branch)
-    if (__builtin_amdgcn_ballot_w64(live) == 0) continue;
+    if (__builtin_amdgcn_ballot_w64(live) == 0) {
+      if (nskip && lane == 0) atomicAdd(nskip, 1u);  // diagnostics: transforms skipped
+      continue;
+    }
     fft_fwd<P>(v, buf, ctx);
     if constexpr (!HOLD) {
       // the opaque copy of the lane offset defeats load CSE with the first read
@@ -228,7 +232,7 @@ template <class P, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
     const float2* __restrict__ A, float2* __restrict__ D, const float* __restrict__ Hy,
     const float2* __restrict__ tw_g, const float* __restrict__ thr_p, int W, int H, int nwx,
-    int nwy, size_t plane, const int* __restrict__ skip) {
+    int nwy, size_t plane, const int* __restrict__ skip, unsigned int* __restrict__ nskip) {
   constexpr int T = P::T, E = P::E;
   extern __shared__ float2 smem[];
   if (skip && *skip) return;  // device-side decision (learned path): not needed
@@ -245,7 +249,7 @@ __global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
   const int x = (b / nwx) * (blockDim.x / T) + slot;
   if (x >= W) return;
   cols_body<P>(A, D, Hy, tw_g, *thr_p, W, H, nwx, nwy, plane, (b % nwx) * W + x,
-               threadIdx.x % T, ttab, buf);
+               threadIdx.x % T, ttab, buf, nskip);
 }
 
 // ---------------------------------------------------------- row inverse

@@ -141,6 +141,8 @@ def lib():
         "glsu_dpra_run": (I, [V, V, I, I, V, PD]),
         "glsu_dpra_run_graph": (I, [V, V, I, I, V, PD]),
         "glsu_dpra_profile": (I, [V, V, I, I, V, PD]),
+        "glsu_dpra_profile_skips": (I, [V, V, I, I, V, PD, ctypes.POINTER(ctypes.c_ulonglong),
+                                        ctypes.POINTER(ctypes.c_ulonglong)]),
         "glsu_dpra_get_state": (I, [V, PF, PF]),
         "glsu_dpra_set_state": (I, [V, PF, PF]),
         "glsu_dpra_copy_z": (I, [V, V]),
@@ -226,7 +228,7 @@ def exported_symbols():
         "glsu_dpra_create", "glsu_dpra_create_f32", "glsu_dpra_create_tuned", "glsu_dpra_get_tuning",
         "glsu_dpra_frame",
         "glsu_dpra_frame_host", "glsu_dpra_update_ref", "glsu_dpra_run", "glsu_dpra_run_graph",
-        "glsu_dpra_profile", "glsu_dpra_time_kernel", "glsu_dpra_get_state", "glsu_dpra_set_state", "glsu_dpra_copy_z", "glsu_dpra_thr",
+        "glsu_dpra_profile", "glsu_dpra_profile_skips", "glsu_dpra_time_kernel", "glsu_dpra_get_state", "glsu_dpra_set_state", "glsu_dpra_copy_z", "glsu_dpra_thr",
         "glsu_dpra_destroy", "glsu_gen_frames", "glsu_dpra_batch_create", "glsu_dpra_batch_run",
         "glsu_dpra_batch_info", "glsu_dpra_batch_get_tuning", "glsu_dpra_batch_get_state",
         "glsu_dpra_batch_destroy", "glsu_unet_create", "glsu_unet_param_count",

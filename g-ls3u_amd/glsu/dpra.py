@@ -115,6 +115,22 @@ class DPRA:
                                       ctypes.c_void_p(out.data_ptr()), ms))
         return list(ms)
 
+    def profile_skips(self, frames, out=None):
+        """profile() that also returns the column transforms skipped because a
+        column's thresholded signal was zero: (ms[4], skipped, transforms)."""
+        dt = self._check_frame(frames, nd=3)
+        T = frames.shape[0]
+        shape = (T, self.height, self.width)
+        if out is None:
+            out = torch.empty(shape, dtype=torch.float32, device=frames.device)
+        check_out(out, shape, torch.float32)
+        ms = (ctypes.c_double * 4)()
+        sk, tot = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+        check(lib().glsu_dpra_profile_skips(self._h, ctypes.c_void_p(frames.data_ptr()), int(T),
+                                            dt, ctypes.c_void_p(out.data_ptr()), ms,
+                                            ctypes.byref(sk), ctypes.byref(tot)))
+        return list(ms), sk.value, tot.value
+
     def time_kernel(self, stage, reps=20):
         """Average ms per launch of one WFF2 kernel (1 rows fwd, 2 columns,
         3 rows inv + update), timed with HIP events on the handle's stream."""

@@ -186,6 +186,14 @@ int glsu_dpra_run_graph(glsu_dpra *h, const void *d_frames, int nframes, int fra
  * DPRA/dpra_cudaf.cu:219-283). */
 int glsu_dpra_profile(glsu_dpra *h, const void *d_frames, int nframes, int frame_dtype,
                       float *d_dphi_all, double *ms);
+/* Diagnostics: glsu_dpra_profile that also counts the column transforms the
+ * column kernel skipped because a column's thresholded signal was zero on
+ * every image row (exact: the skipped transform would add zero).  *skipped:
+ * that count; *transforms: the count without skipping, nframes * nwx * width
+ * * nwy.  No reference counterpart. */
+int glsu_dpra_profile_skips(glsu_dpra *h, const void *d_frames, int nframes, int frame_dtype,
+                            float *d_dphi_all, double *ms, unsigned long long *skipped,
+                            unsigned long long *transforms);
 /* Diagnostics: average device time (ms) of one launch of a WFF2 kernel, from
  * `reps` back-to-back launches between two HIP events on the handle's stream.
  * stage: 1 row-forward, 2 column, 3 row-inverse+update.  Re-runs the kernel on

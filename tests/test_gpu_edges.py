@@ -137,3 +137,23 @@ def test_cpp_aia_and_wfr_classes_example_runs():
     say(r.stdout.strip())
     assert r.returncode == 0, r.stdout + r.stderr
     assert "example_classes: ok" in r.stdout
+
+
+def test_profile_counts_skipped_column_transforms(oracle):
+    """glsu_dpra_profile_skips: with thr = 0 every thresholded sample survives
+    (no transform is skipped); with a threshold above every |sample| every
+    column is zero after thresholding (all are skipped, and the output is
+    the zero field's phase).  The default threshold lies in between."""
+    W, H, T = 160, 96, 2
+    frames, phi0 = oracle.gen_sequence(W, H, T, 5, dtype=np.float32)
+    fr = torch.from_numpy(frames).cuda()
+    total = T * 5 * W * 5
+    for thr, expect in ((0.0, 0), (1e30, total), (-1.0, None)):
+        d = glsu.DPRA(phi0, params=glsu.wff_params(10.0, -0.2, 0.2, 0.1, thr=thr))
+        ms, skipped, transforms = d.profile_skips(fr)
+        say(f"[skips] thr={thr}: {skipped} of {transforms} column transforms skipped")
+        assert transforms == total
+        if expect is not None:
+            assert skipped == expect
+        else:
+            assert 0 <= skipped < total

@@ -130,6 +130,7 @@ def lib():
         "glsu_wft2_compute_wfr": (I, [V, V, ctypes.POINTER(WfrResults), PD]),
         "glsu_wft2_compute_wfr_host": (I, [V, V, V, PD]),
         "glsu_wft2_time_kernel": (I, [V, V, I, I, PD]),
+        "glsu_wft2_compute_noalloc": (I, [I, I, PP, V, V, V]),
         "glsu_dpra_create": (I, [PV, PD, I, I, I, PP, V]),
         "glsu_dpra_create_f32": (I, [PV, PF, I, I, I, PP, V]),
         "glsu_dpra_create_tuned": (I, [PV, PD, I, I, I, PP, V, PT]),
@@ -224,7 +225,7 @@ def exported_symbols():
         "glsu_wff_params_dpra_reference", "glsu_wft2_create", "glsu_wft2_create_tuned",
         "glsu_wft2_compute", "glsu_wft2_compute_host", "glsu_wft2_info", "glsu_wft2_get_tuning",
         "glsu_wft2_destroy", "glsu_wft2_compute_wfr", "glsu_wft2_compute_wfr_host",
-        "glsu_wft2_time_kernel",
+        "glsu_wft2_time_kernel", "glsu_wft2_compute_noalloc",
         "glsu_dpra_create", "glsu_dpra_create_f32", "glsu_dpra_create_tuned", "glsu_dpra_get_tuning",
         "glsu_dpra_frame",
         "glsu_dpra_frame_host", "glsu_dpra_update_ref", "glsu_dpra_run", "glsu_dpra_run_graph",

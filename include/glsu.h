@@ -138,6 +138,24 @@ int glsu_wft2_info(glsu_wft2 *h, int *nx, int *ny, int *sx, int *sy, int *nwx, i
                    double *thr);
 int glsu_wft2_destroy(glsu_wft2 *h);
 
+/* One WFF-mode WFT2 (constructor + operator()) of a height*width complex64
+ * field, d_f -> d_filtered (device, row-major), enqueued on hip_stream (NULL:
+ * the null stream; the current device must be the stream's), with the plans
+ * and launch shapes glsu_wft2_create picks by default.  It allocates nothing:
+ * every intermediate (window spectra, twiddles, threshold, partial sums, the
+ * row and column planes) lives in one file-scope device array of 96 MiB that
+ * each device holds from the library's first use of it until the process
+ * ends; the tables are computed on the device on every call.  thr < 0 is
+ * computed from this call's field (nothing is cached between calls).  Fields
+ * whose planes do not fit (2 * nwx * height * ceil8(width) * 8 bytes plus
+ * under 1 MiB; e.g. 1024x1024 with up to 5 wx values) return
+ * GLSU_ERR_UNSUPPORTED.  Calls that overlap on one device share the array and
+ * race: order them on one stream (or with events).  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops). */
+int glsu_wft2_compute_noalloc(int width, int height, const glsu_wff_params *p, const void *d_f,
+                              void *d_filtered, void *hip_stream);
+
 /* ---- DPRA ------------------------------------------------------------------
  * Replaces DPRA_CUDAF::DPRA_CUDAF(const float *Phi0, W, H, rr),
  * DPRA/dpra_cudaf.h:28-30, DPRA/dpra_cudaf.cu:96-164, and the FP64 CPU

@@ -611,14 +611,16 @@ def main():
     gpu_out = out.cpu().numpy() if not args.no_cpu else None  # the last timed step's output
 
     # --- roofline of the dominant kernel (the column WFF2 kernel) ----------------
-    # HIP events on the handle's stream around each kernel of every frame of one
-    # more pass of the same sequence from phi0 (after the timed region); the
-    # column kernel's average over the pass is its average launch duration in
-    # the chain.  Also: 20 back-to-back launches on the state the pass ends in.
+    # One more pass of the same sequence from phi0 (after the timed region) with
+    # HIP events on the handle's stream around each kernel of every frame (these
+    # intervals also hold the dispatch gap before each kernel) and the count of
+    # skipped column transforms; then 20 back-to-back launches of the column
+    # kernel on the state the pass ends in, between two events: its average
+    # launch duration.
     reset()
     kms, skipped, transforms = d.profile_skips(frames)
-    col_ms = kms[2] / T
-    col_b2b_ms = d.time_kernel(2, reps=20)
+    col_chain_ms = kms[2] / T
+    col_ms = d.time_kernel(2, reps=20)
     info = glsu.WFT2(W, H, params=params, tuning=tuning).info()
     nwx, nwy, Ny = info["nwx"], info["nwy"], info["ny"]
     flops = col_kernel_flops(W, H, nwx, nwy, Ny)
@@ -635,12 +637,15 @@ def main():
         "bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
         "kernel": "wff_cols_kernel", "avg_ms": round(col_ms, 5),
-        "avg_ms_source": "HIP events on the handle's stream around the column kernel of every "
-                         f"frame of one pass of the {T}-frame sequence from phi0",
-        "back_to_back_ms": round(col_b2b_ms, 5),
+        "avg_ms_source": "HIP events on the handle's stream around 20 back-to-back launches "
+                         f"on the state after one pass of the {T}-frame sequence",
+        "per_frame_in_chain_ms": round(col_chain_ms, 5),
+        "per_frame_in_chain_note": "events around the column kernel of every frame of that "
+                                   "pass, each interval including the dispatch gap before it",
         "skipped_transforms": {"skipped": skipped, "of": transforms,
                                "frac": round(skipped / max(transforms, 1), 4),
-                               "executed_TFLOPs": round((flops - skip_flops) / (col_ms * 1e-3) / 1e12, 2),
+                               "executed_TFLOPs_per_frame_in_chain": round((flops - skip_flops) / (col_chain_ms * 1e-3)
+                                                         / 1e12, 2),
                                "note": "forward transforms of a (wx, column, wy) whose thresholded "
                                        "column is zero on every image row add exactly nothing and "
                                        "are skipped; achieved counts them as done"},

@@ -829,44 +829,62 @@ extern "C" int glsu_dpra_profile_skips(glsu_dpra* h, const void* frames, int nfr
   return GLSU_OK;
 }
 
+// Events are recorded for up to kProfBatch frames before the host waits for
+// them, so the kernels run back to back as in the chain (a host wait after
+// every frame would leave the device idle between frames and slow the next
+// kernels' start).
 static int dpra_profile_impl(glsu_dpra* h, const void* frames, int nframes, int dtype,
                              float* dphi_all, double* ms) {
   if (!h || !frames || !ms || nframes < 0) return fail(GLSU_ERR_ARG, "invalid argument");
   const int es = elem_size(dtype);
   if (!es) return fail(GLSU_ERR_ARG, "unknown frame dtype %d", dtype);
   const size_t n = (size_t)h->W * h->H;
-  hipEvent_t ev[5];
-  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  constexpr int kProfBatch = 64;
+  std::vector<hipEvent_t> ev(5 * kProfBatch, nullptr);
+  int rc = GLSU_OK;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) rc = fail(GLSU_ERR_DEVICE, "hipEventCreate failed");
   for (int k = 0; k < 4; ++k) ms[k] = 0;
   WffPlan& p = h->p;
-  for (int i = 0; i < nframes; ++i) {
-    const char* f = (const char*)frames + (size_t)i * n * es;
-    HIP_TRY(hipEventRecord(ev[0], h->s));
-    HIP_TRY(launch_dpra_ls(f, dtype, h->phiref, h->z, h->W, h->H, h->W, h->s, h->p.ls_tile));
-    if (!p.thr_ready) {
-      HIP_TRY(launch_thr(p, h->z, h->s));
-      p.thr_ready = true;
+  auto run = [&]() -> int {
+    for (int i0 = 0; i0 < nframes; i0 += kProfBatch) {
+      const int nb = std::min(kProfBatch, nframes - i0);
+      for (int b = 0; b < nb; ++b) {
+        const int i = i0 + b;
+        hipEvent_t* e = &ev[5 * b];
+        const char* f = (const char*)frames + (size_t)i * n * es;
+        HIP_TRY(hipEventRecord(e[0], h->s));
+        HIP_TRY(launch_dpra_ls(f, dtype, h->phiref, h->z, h->W, h->H, h->W, h->s, h->p.ls_tile));
+        if (!p.thr_ready) {
+          HIP_TRY(launch_thr(p, h->z, h->s));
+          p.thr_ready = true;
+        }
+        HIP_TRY(hipEventRecord(e[1], h->s));
+        HIP_TRY(launch_rows_fwd(p, h->z, h->s));
+        HIP_TRY(hipEventRecord(e[2], h->s));
+        HIP_TRY(launch_cols(p, h->s));
+        HIP_TRY(hipEventRecord(e[3], h->s));
+        UpdateArgs u{h->phiref, h->dref_hi, h->dref_lo, h->phicurr, h->dphi_hi, h->dphi_lo,
+                     dphi_all ? dphi_all + (size_t)i * n : nullptr, nullptr};
+        HIP_TRY(launch_rows_inv_update(p, u, h->s));
+        HIP_TRY(hipEventRecord(e[4], h->s));
+        h->fresh = true;
+        if (i % h->rr == 0) dpra_swap(h);
+      }
+      HIP_TRY(hipEventSynchronize(ev[5 * (nb - 1) + 4]));
+      for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < 4; ++k) {
+          float t = 0;
+          HIP_TRY(hipEventElapsedTime(&t, ev[5 * b + k], ev[5 * b + k + 1]));
+          ms[k] += t;
+        }
     }
-    HIP_TRY(hipEventRecord(ev[1], h->s));
-    HIP_TRY(launch_rows_fwd(p, h->z, h->s));
-    HIP_TRY(hipEventRecord(ev[2], h->s));
-    HIP_TRY(launch_cols(p, h->s));
-    HIP_TRY(hipEventRecord(ev[3], h->s));
-    UpdateArgs u{h->phiref, h->dref_hi, h->dref_lo, h->phicurr, h->dphi_hi, h->dphi_lo,
-                 dphi_all ? dphi_all + (size_t)i * n : nullptr, nullptr};
-    HIP_TRY(launch_rows_inv_update(p, u, h->s));
-    HIP_TRY(hipEventRecord(ev[4], h->s));
-    h->fresh = true;
-    if (i % h->rr == 0) dpra_swap(h);
-    HIP_TRY(hipEventSynchronize(ev[4]));
-    for (int k = 0; k < 4; ++k) {
-      float t = 0;
-      HIP_TRY(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
-      ms[k] += t;
-    }
-  }
-  for (auto& e : ev) hipEventDestroy(e);
-  return GLSU_OK;
+    return GLSU_OK;
+  };
+  if (!rc) rc = run();
+  for (auto& e : ev)
+    if (e) hipEventDestroy(e);
+  return rc;
 }
 
 extern "C" int glsu_dpra_time_kernel(glsu_dpra* h, int stage, int reps, double* ms) {

@@ -444,7 +444,9 @@ def bench_batch(args, glsu, torch, np):
         "config": {"workload": f"batched FP32 DPRA+WFF2: {n} sequences, sizes uniform over "
                                f"{{256..2048}}^2, {T} frames each, one batched run per step, "
                                "WFF2 sigma=10, [-0.2,0.2] step 0.1 (25 pairs), per-sequence "
-                               "amp0 in U[2,5] (+0.1 t) and noise in U[5,25]",
+                               "amp0 in U[2,5] (+0.1 t) and noise in U[5,25]; each step "
+                               "continues the sequences from the state the previous step left "
+                               "(repeated operator() calls)",
                    "name": "batch64", "sizes": sizes, "plan_classes": b.info()["classes"],
                    "mode": args.mode},
         "sequential_single_handles_frames_per_s": round(seq_fps, 2),
@@ -458,6 +460,45 @@ def bench_batch(args, glsu, torch, np):
                              "reference padding) over all sequences per step; per-kernel "
                              "figures are those of the plain1024 workload"},
     }
+    if not args.no_cpu:
+        # a documented per-sequence subset: the first sequence of each size up to
+        # 1024^2, first 2 frames, FP64 oracle with the reference's loop (CPU
+        # baseline) against the batched outputs of the same frames (parity)
+        from oracle import oracle as O
+        hi = host_info()
+        sub = [sizes.index(S) for S in BATCH_SIZES if S <= 1024 and S in sizes]
+        k = min(2, T)
+        # a new batch of those sequences from their phi0 (the timed batch's state
+        # has moved on, and its outputs were overwritten by the single-handle runs)
+        bsub = glsu.DPRABatch([phis[i] for i in sub], 1, params=params,
+                              tunings=[tunings[i] for i in sub] if tunings else None)
+        osub = bsub([frames[i][:k].contiguous() for i in sub], graph=graph)
+        t0 = time.perf_counter()
+        px_done = 0
+        rows = []
+        for j, i in enumerate(sub):
+            S = sizes[i]
+            host = frames[i][:k].cpu().numpy().astype(np.float64)
+            od = O.DPRA(phis[i], sigx=wff[0], wxl=wff[1], wxh=wff[2], wxi=wff[3],
+                        nthreads=hi["threads"])
+            ref = od.run(host)
+            px_done += k * S * S
+            got = osub[j].cpu().numpy()
+            for t in range(k):
+                r, m = O.wrapped_err(got[t], ref[t])
+                rows.append({"seq": i, "size": S, "frame": t, "rms_rad": r, "max_rad": m})
+        cdt = time.perf_counter() - t0
+        px_all = sum(S * S for S in sizes) * T
+        line["cpu_baseline"] = {
+            "value": round(px_done / cdt / px_all * n * T, 4), "unit": "frames/s",
+            "cores": hi["threads"], "kind": "port",
+            "sample": f"first {k} frames of sequences {sub} (sizes {[sizes[i] for i in sub]}), "
+                      f"FP64 oracle with the reference's loop (OpenMP over wy), {cdt:.1f} s; "
+                      "scaled to the whole workload by pixel count",
+            "host": hi}
+        line["parity"] = {"frames": rows, "rms_rad_max": max(r["rms_rad"] for r in rows),
+                          "note": "batched output of the first frames of a sequence of each "
+                                  "size up to 1024^2 against the FP64 oracle chain"}
     print(json.dumps(line), flush=True)
 
 

@@ -132,6 +132,12 @@ extern "C" int glsu_fft_plans(int* ids, int* lanes, int* elems, int* lengths, in
   return GLSU_OK;
 }
 
+extern "C" int glsu_fft_plan_exchange(int id) {
+  for (auto& p : fft_plans())
+    if (p.id == id) return p.exchange;
+  return fail(GLSU_ERR_ARG, "unknown FFT plan id %d", id);
+}
+
 // Padded size rule of the reference (WFT/WFT2_CPU.cpp:710-724 + WFT/WFT.h:18-32):
 // the constructor fails when OPT(H + 2s) or OPT(W + 2s) exceeds 4096.  The
 // library pads differently (any N >= size + s is exact) but keeps the same

@@ -11,7 +11,9 @@
 //   through LDS laid out as [k1][position].  In every stage a lane owns ONE
 //   butterfly position c and E/R values of k1, so its twiddles are a few
 //   per-lane constants and every LDS address is lane_base + compile-time
-//   offset (no per-element index arithmetic to keep live).
+//   offset (no per-element index arithmetic to keep live).  With 64 lanes and
+//   a first radix of 4, the first stage's inputs are gathered across the
+//   wavefront's four 16-lane rows with permlane swaps instead (FftPlan::XL0).
 // The last stage leaves the spectrum in registers ("spectral layout": lane
 //   c + (T/R) g, register i + m (E/R) holds X[(g + R i) + E (c + m T/R)]).
 // The inverse runs the stages in mirror order (inverse butterflies, conjugate
@@ -296,7 +298,73 @@ struct FftPlan {
     const int nb = E_ / RL, i = j % nb, m = j / nb;
     return RL * i + E_ * (T_ / RL) * m;
   }
+  // The transition into the first cross-lane stage (and, in the inverse, out
+  // of it) moves data between the wavefront's four 16-lane rows in registers
+  // (v_permlane32_swap / v_permlane16_swap, xl_gather / xl_scatter) instead of
+  // an LDS write + read: one LDS round trip fewer per transform, the same
+  // arithmetic.  Needs a whole wavefront per transform and a first radix of 4
+  // (the stage's lane groups are then exactly the rows).
+  static constexpr bool XL0 = T_ == 64 && Rad::at(0) == 4 && NS > 1;
 };
+
+// ----------------------------------------------- row exchange in registers
+// Swap of lanes 32..63 of a with lanes 0..31 of b (v_permlane32_swap), and of
+// the odd 16-lane rows of a with the even rows of b (v_permlane16_swap).
+__device__ __forceinline__ void swap_half32(float& a, float& b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false,
+                                                  false);
+  a = __uint_as_float(r[0]);
+  b = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void swap_rows16(float& a, float& b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false,
+                                                  false);
+  a = __uint_as_float(r[0]);
+  b = __uint_as_float(r[1]);
+}
+// 4 x 4 transpose between the rows of the wavefront (row g = lanes 16g..16g+15)
+// and four registers: afterwards register m of row g holds what register g of
+// row m held (lane within the row unchanged).  The transpose is its own inverse.
+__device__ __forceinline__ void xpose_rows4(float2& r0, float2& r1, float2& r2, float2& r3) {
+  swap_half32(r0.x, r2.x);
+  swap_half32(r0.y, r2.y);
+  swap_half32(r1.x, r3.x);
+  swap_half32(r1.y, r3.y);
+  swap_rows16(r0.x, r1.x);
+  swap_rows16(r0.y, r1.y);
+  swap_rows16(r2.x, r3.x);
+  swap_rows16(r2.y, r3.y);
+}
+
+// Stage-0 gather in registers (forward): lane n1 = c + 16 g' holds the
+// twiddled x[k1] of its column n1 in register k1.  Stage 0 wants, in lane
+// (c, g), register i + m NB = element k1 = g + 4 i of column c + 16 m.
+// Group i (registers 4i .. 4i+3) transposed across rows gives exactly that.
+template <class P>
+__device__ __forceinline__ void xl_gather(float2 (&v)[P::E]) {
+  constexpr int E = P::E, NB = E / 4;
+  float2 w[E];
+  sfor<0, NB>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    xpose_rows4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+    sfor<0, 4>([&](auto mc) { w[i + decltype(mc)::value * NB] = v[4 * i + decltype(mc)::value]; });
+  });
+  sfor<0, E>([&](auto kc) { v[decltype(kc)::value] = w[decltype(kc)::value]; });
+}
+// Inverse of xl_gather.
+template <class P>
+__device__ __forceinline__ void xl_scatter(float2 (&v)[P::E]) {
+  constexpr int E = P::E, NB = E / 4;
+  float2 w[E];
+  sfor<0, NB>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    float2 r[4];
+    sfor<0, 4>([&](auto mc) { r[decltype(mc)::value] = v[i + decltype(mc)::value * NB]; });
+    xpose_rows4(r[0], r[1], r[2], r[3]);
+    sfor<0, 4>([&](auto mc) { w[4 * i + decltype(mc)::value] = r[decltype(mc)::value]; });
+  });
+  sfor<0, E>([&](auto kc) { v[decltype(kc)::value] = w[decltype(kc)::value]; });
+}
 
 // Per-lane constants of one plan: transition twiddles W_N^{lane k1} and the
 // stage twiddles W_{L R}^{(c mod L) m}, read once from the W_N^k table.
@@ -359,23 +427,26 @@ struct FftLane {
   }
 };
 
-template <class P, int S, bool INV, class Ctx>
+template <class P, int S, bool INV, class Ctx, bool REG_IN = false>
 __device__ __forceinline__ void stage_fwd(float2 (&v)[P::E], float2* buf, const Ctx& L_) {
   // forward Stockham stage S: read z[k1][c + m T/R] -> twiddle -> DFT_R -> write
+  // (REG_IN: the stage's inputs are already in v, gathered in registers)
   constexpr int T = P::T, E = P::E, ROW = P::ROW;
   constexpr int R = P::Rad::at(S), L = P::Rad::prod_before(S), NB = E / R;
   constexpr bool LAST = (S == P::NS - 1);
   const int lane = L_.lane;
   const int c = lane % (T / R), g = lane / (T / R);
-  const float2* rb = buf + g * ROW;
-  sfor<0, R>([&](auto mc) {
-    constexpr int m = decltype(mc)::value;
-    const int o = P::phys(c + m * (T / R));
-    sfor<0, NB>([&](auto ic) {
-      constexpr int i = decltype(ic)::value;
-      v[i + m * NB] = rb[R * i * ROW + o];
+  if constexpr (!REG_IN) {
+    const float2* rb = buf + g * ROW;
+    sfor<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      const int o = P::phys(c + m * (T / R));
+      sfor<0, NB>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        v[i + m * NB] = rb[R * i * ROW + o];
+      });
     });
-  });
+  }
   wave_sync();
   float2* wb = buf + g * ROW;
   const int wbase = (c / L) * L * R + (c % L);
@@ -398,9 +469,10 @@ __device__ __forceinline__ void stage_fwd(float2 (&v)[P::E], float2* buf, const 
   if constexpr (!LAST) wave_sync();
 }
 
-template <class P, int S, class Ctx>
+template <class P, int S, class Ctx, bool REG_OUT = false>
 __device__ __forceinline__ void stage_inv(float2 (&v)[P::E], float2* buf, const Ctx& L_) {
   // mirror of stage_fwd: read outputs -> inverse DFT_R -> conj twiddle -> write inputs
+  // (REG_OUT: the inputs stay in v, in the stage's register order, for xl_scatter)
   constexpr int T = P::T, E = P::E, ROW = P::ROW;
   constexpr int R = P::Rad::at(S), L = P::Rad::prod_before(S), NB = E / R;
   constexpr bool LAST = (S == P::NS - 1);
@@ -429,10 +501,11 @@ __device__ __forceinline__ void stage_inv(float2 (&v)[P::E], float2* buf, const 
       constexpr int m = decltype(mc)::value;
       float2 a = u[m];
       if constexpr (L > 1 && m > 0) a = cmulc(a, L_.ts[S][m]);
-      wb[R * i * ROW + P::phys(c + m * (T / R))] = a;
+      if constexpr (REG_OUT) v[i + m * NB] = a;
+      else wb[R * i * ROW + P::phys(c + m * (T / R))] = a;
     });
   });
-  wave_sync();
+  if constexpr (!REG_OUT) wave_sync();
 }
 
 template <class P, int S, class Ctx, int LO = 0>
@@ -455,17 +528,27 @@ __device__ __forceinline__ void fft_fwd(float2 (&v)[P::E], float2* buf, const Ct
     __builtin_amdgcn_sched_barrier(0);
   }
   Dft<E, false>::run(v);
-  float2* wb = buf + P::phys(L_.lane);
-  sfor<0, E>([&](auto kc) {
-    constexpr int k = decltype(kc)::value;
-    if constexpr (k == 0) {
-      wb[0] = v[0];
-    } else {
-      wb[k * ROW] = cmul(v[k], Ctx::kTTLds ? tw[k] : L_.tw_trans(tr, k));
-    }
-  });
-  wave_sync();
-  sfor<0, P::NS>([&](auto sc) { stage_fwd<P, decltype(sc)::value, false>(v, buf, L_); });
+  if constexpr (P::XL0) {
+    sfor<1, E>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      v[k] = cmul(v[k], Ctx::kTTLds ? tw[k] : L_.tw_trans(tr, k));
+    });
+    xl_gather<P>(v);
+    stage_fwd<P, 0, false, Ctx, true>(v, buf, L_);
+    sfor<1, P::NS>([&](auto sc) { stage_fwd<P, decltype(sc)::value, false>(v, buf, L_); });
+  } else {
+    float2* wb = buf + P::phys(L_.lane);
+    sfor<0, E>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      if constexpr (k == 0) {
+        wb[0] = v[0];
+      } else {
+        wb[k * ROW] = cmul(v[k], Ctx::kTTLds ? tw[k] : L_.tw_trans(tr, k));
+      }
+    });
+    wave_sync();
+    sfor<0, P::NS>([&](auto sc) { stage_fwd<P, decltype(sc)::value, false>(v, buf, L_); });
+  }
 }
 
 // Inverse DFT (unnormalised, e^{+i}): spectral layout in, spatial layout out.
@@ -481,17 +564,26 @@ __device__ __forceinline__ void fft_inv(float2 (&v)[P::E], float2* buf, const Ct
     sfor<1, E>([&](auto kc) { tw[decltype(kc)::value] = L_.tw_trans(tr, decltype(kc)::value); });
     __builtin_amdgcn_sched_barrier(0);
   }
-  stage_inv<P, 0>(v, buf, L_);
-  const float2* rb = buf + P::phys(L_.lane);
-  sfor<0, E>([&](auto kc) {
-    constexpr int k = decltype(kc)::value;
-    const float2 a = rb[k * ROW];
-    if constexpr (k == 0) {
-      v[0] = a;
-    } else {
-      v[k] = cmulc(a, Ctx::kTTLds ? tw[k] : L_.tw_trans(tr, k));
-    }
-  });
+  if constexpr (P::XL0) {
+    stage_inv<P, 0, Ctx, true>(v, buf, L_);
+    xl_scatter<P>(v);
+    sfor<1, E>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      v[k] = cmulc(v[k], Ctx::kTTLds ? tw[k] : L_.tw_trans(tr, k));
+    });
+  } else {
+    stage_inv<P, 0>(v, buf, L_);
+    const float2* rb = buf + P::phys(L_.lane);
+    sfor<0, E>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      const float2 a = rb[k * ROW];
+      if constexpr (k == 0) {
+        v[0] = a;
+      } else {
+        v[k] = cmulc(a, Ctx::kTTLds ? tw[k] : L_.tw_trans(tr, k));
+      }
+    });
+  }
   Dft<E, true>::run(v);
 }
 

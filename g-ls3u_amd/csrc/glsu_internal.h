@@ -13,6 +13,7 @@ struct FftPlanInfo {
   int id;
   int T, E, N;
   const char* radices;  // cross-lane radices, for reports
+  int exchange;         // 0 every stage through LDS, 1 first stage through registers
 };
 const std::vector<FftPlanInfo>& fft_plans();
 // Plans with N >= need, sorted by N (then by fewest LDS stages).

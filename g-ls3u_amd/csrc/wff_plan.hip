@@ -3,7 +3,8 @@
 
 namespace glsu {
 
-#define GLSU_PLAN_INFO(id, T, E, ...) {id, T, E, T * E, #__VA_ARGS__},
+#define GLSU_PLAN_INFO(id, T, E, ...) \
+  {id, T, E, T * E, #__VA_ARGS__, FftPlan<T, E, __VA_ARGS__>::XL0 ? 1 : 0},
 const std::vector<FftPlanInfo>& fft_plans() {
   static const std::vector<FftPlanInfo> v = {GLSU_PLANS(GLSU_PLAN_INFO)};
   return v;

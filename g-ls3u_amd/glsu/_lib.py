@@ -175,6 +175,7 @@ def lib():
         "glsu_read_complex_text": (I, [ctypes.c_char_p, PI, PI, V, ctypes.c_size_t]),
         "glsu_write_complex_text": (I, [ctypes.c_char_p, I, I, V, I]),
         "glsu_fft_plans": (I, [PI, PI, PI, PI, I, PI]),
+        "glsu_fft_plan_exchange": (I, [I]),
         "glsu_phase_to_u8": (I, [V, V, ctypes.c_size_t, V]),
         "glsu_aia_create": (I, [PV, I, I, I, V]),
         "glsu_aia_run": (I, [V, V, I, V, I, U64, I, D, I, V, PI, PD, PD]),
@@ -237,7 +238,7 @@ def exported_symbols():
         "glsu_unet_loss_grad", "glsu_unet_adam_step", "glsu_unet_destroy",
         "glsu_dpra_frame_learned", "glsu_dpra_copy_net", "glsu_dpra_learned_stats",
         "glsu_read_matrix_bin", "glsu_write_matrix_bin", "glsu_read_complex_text",
-        "glsu_write_complex_text", "glsu_fft_plans", "glsu_phase_to_u8",
+        "glsu_write_complex_text", "glsu_fft_plans", "glsu_fft_plan_exchange", "glsu_phase_to_u8",
         "glsu_aia_create", "glsu_aia_run", "glsu_aia_run_host", "glsu_aia_info", "glsu_aia_time_iter", "glsu_aia_destroy",
     ]
 
@@ -249,4 +250,6 @@ def fft_plans():
     e = (ctypes.c_int * 64)()
     ln = (ctypes.c_int * 64)()
     check(lib().glsu_fft_plans(ids, t, e, ln, 64, ctypes.byref(n)))
-    return [dict(id=ids[i], T=t[i], E=e[i], N=ln[i]) for i in range(n.value)]
+    # exchange: 0 every cross-lane stage through LDS, 1 the first through registers
+    return [dict(id=ids[i], T=t[i], E=e[i], N=ln[i], exchange=lib().glsu_fft_plan_exchange(ids[i]))
+            for i in range(n.value)]

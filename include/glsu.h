@@ -377,6 +377,13 @@ int glsu_gen_frames(float *d_out, int width, int height, int t0, int nframes, ui
 /* FFT plans compiled into the library: fills up to cap entries of
  * (id, T, E, N) and returns the count in *count. */
 int glsu_fft_plans(int *ids, int *lanes, int *elems, int *lengths, int cap, int *count);
+/* How FFT plan `id` moves data between the lanes of a transform: 0 = every
+ * cross-lane stage through LDS; 1 = the transition into the first stage (and
+ * out of it in the inverse) through registers (v_permlane32/16_swap between
+ * the wavefront's 16-lane rows), the other stages through LDS.  The kind
+ * changes no arithmetic.  Returns the kind, or GLSU_ERR_ARG for an unknown
+ * id.  No reference counterpart (plan choice is internal to this library). */
+int glsu_fft_plan_exchange(int id);
 
 #ifdef __cplusplus
 }

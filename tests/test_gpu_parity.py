@@ -104,7 +104,8 @@ def test_every_fft_plan_matches_oracle(oracle):
     for p in plans:
         W = min(p["N"] - 6, 300)
         H = max(min(p["N"] - 6, 120), 16)
-        say(f"[plan {p['id']}] T={p['T']} E={p['E']} N={p['N']} image {W}x{H}")
+        say(f"[plan {p['id']}] T={p['T']} E={p['E']} N={p['N']} image {W}x{H} "
+            f"exchange={p['exchange']}")
         rng = np.random.default_rng(p["id"])
         ph = 0.6 * np.arange(W)[None, :] + 0.25 * np.arange(H)[:, None] + rng.normal(0, 0.3, (H, W))
         f = np.exp(1j * ph).astype(np.complex64)

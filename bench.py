@@ -530,20 +530,32 @@ def _opt(n):
         m += 1
 
 
-def _traffic(W, H, T, tuning):
-    """HBM bytes per launch of the column kernel from the committed rocprofv3
-    --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration), when it holds an entry for this size and tuning."""
+def _col_entry(W, H, tuning):
+    """The committed rocprofv3 summary entry for the column kernel at this size
+    and with this kernel's own launch choices (plany, cols_waves; the other
+    tuning fields pick other kernels), or None."""
     path = os.path.join(ROOT, "profiles", f"wff_cols_{W}x{H}_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
-        return None
+        return None, path
     for e in t.get("entries", []):
-        if e.get("tuning") == tuning and e.get("size") == [H, W]:
-            return {"bytes": int(e["hbm_bytes_per_launch"]), "source": os.path.relpath(path, ROOT)}
-    return None
+        et = e.get("tuning", {})
+        if e.get("size") == [H, W] and all(et.get(k) == tuning.get(k)
+                                           for k in ("plany", "cols_waves")):
+            return e, path
+    return None, path
+
+
+def _traffic(W, H, T, tuning):
+    """HBM bytes per launch of the column kernel from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), when it holds an entry for this size and tuning."""
+    e, path = _col_entry(W, H, tuning)
+    if e is None:
+        return None
+    return {"bytes": int(e["hbm_bytes_per_launch"]), "source": os.path.relpath(path, ROOT)}
 
 
 def main():
@@ -878,16 +890,10 @@ def one_step_parity(np, O, glsu, torch, phi0, frames, host, nf, params, tuning, 
 def _profiled(W, H, tuning):
     """Average column-kernel duration from the committed rocprofv3 kernel
     trace summary, when it holds an entry for this size and tuning."""
-    path = os.path.join(ROOT, "profiles", f"wff_cols_{W}x{H}_traffic.json")
-    try:
-        with open(path) as f:
-            t = json.load(f)
-    except (OSError, ValueError):
+    e, path = _col_entry(W, H, tuning)
+    if e is None or "trace_avg_ms" not in e:
         return None
-    for e in t.get("entries", []):
-        if e.get("tuning") == tuning and e.get("size") == [H, W] and "trace_avg_ms" in e:
-            return {"avg_ms": float(e["trace_avg_ms"]), "source": os.path.relpath(path, ROOT)}
-    return None
+    return {"avg_ms": float(e["trace_avg_ms"]), "source": os.path.relpath(path, ROOT)}
 
 
 if __name__ == "__main__":

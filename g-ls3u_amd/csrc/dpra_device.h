@@ -88,19 +88,26 @@ __device__ __forceinline__ float2 dpra_pixel(const float* sI, const float* sS, c
     Se += e; See += e * e; Sss += sq * sq; SI += Iq; SIe += Iq * e;
     Ss.add(sq); Ses.add(e * sq); SIs.add(Iq * sq);
   };
+  // the centre's own term has d = 0 exactly (e = s' = 0): it adds I_p to SI and nothing
+  // else (through the difference formula it would add the rounding residue of sh)
+  auto centre = [&]() { SI += sI[ly * pitch + lx]; };
   if (x > 0 && x < W - 1 && y > 0 && y < H - 1) {  // interior: full neighbourhood
     n = 9;
 #pragma unroll
     for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-      for (int dx = -1; dx <= 1; ++dx) neighbour(dy, dx);
+      for (int dx = -1; dx <= 1; ++dx) {
+        if (dy == 0 && dx == 0) centre();
+        else neighbour(dy, dx);
+      }
   } else {  // image border: clipped neighbourhood (4 or 6 pixels)
     for (int dy = -1; dy <= 1; ++dy)
       for (int dx = -1; dx <= 1; ++dx) {
         const int gx = x + dx, gy = y + dy;
         if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
         ++n;
-        neighbour(dy, dx);
+        if (dy == 0 && dx == 0) centre();
+        else neighbour(dy, dx);
       }
   }
   // cos/sin of phi_p by the double-angle formulas

@@ -42,6 +42,13 @@ def test_version_and_plans():
     assert all(p["N"] == p["T"] * p["E"] for p in plans)
     assert any(p["N"] >= 1024 + 30 for p in plans)
     assert any(p["N"] >= 2048 + 36 for p in plans)
+    # the first cross-lane stage goes through permlane swaps exactly for the
+    # 64-lane plans whose first radix is 4 (the radix-8 and narrower plans use LDS)
+    by_id = {p["id"]: p for p in plans}
+    assert by_id[4]["exchange"] == 1 and by_id[4]["N"] == 1280
+    assert by_id[1]["exchange"] == 0  # 64 x 8, radices 8 x 8
+    assert all(p["exchange"] == 0 for p in plans if p["T"] != 64)
+    assert glsu.lib().glsu_fft_plan_exchange(999) == glsu._lib.ERR_ARG
 
 
 def test_param_helpers():

@@ -179,14 +179,45 @@ struct Dft<5, INV> {
   }
 };
 
-// generic: prime sizes by direct O(N^2) sums, composite N = P*Q by one
-// decimation step X[k1 + P k2] = sum_n2 W_Q^{n2 k2} W_N^{n2 k1} sum_n1 x[n2 + Q n1] W_P^{n1 k1}
+constexpr int ct_gcd(int a, int b) { return b == 0 ? a : ct_gcd(b, a % b); }
+constexpr int ct_modinv(int a, int m) {  // a^-1 mod m for gcd(a, m) = 1 (m small)
+  for (int x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  return m == 1 ? 0 : -1;
+}
+
+// generic: prime sizes by direct O(N^2) sums; composite N = P*Q with gcd(P, Q)
+// = 1 by the prime-factor (Good-Thomas) mapping, which needs no twiddles:
+//   n = (Q n1 + P n2) mod N,  k = (Q a k1 + P b k2) mod N,  a = Q^-1 mod P, b = P^-1 mod Q,
+//   X[k] = sum_n2 W_Q^{n2 k2} sum_n1 x[n] W_P^{n1 k1};
+// other composites by one Cooley-Tukey decimation step
+//   X[k1 + P k2] = sum_n2 W_Q^{n2 k2} W_N^{n2 k1} sum_n1 x[n2 + Q n1] W_P^{n1 k1}.
+// Published algorithms: Good (1958) / Thomas (1963), Cooley-Tukey (1965).
 template <int N, bool INV>
 struct Dft {
   __host__ __device__ __forceinline__ static void run(float2* x) {
     constexpr int P = pick_factor(N);
     constexpr int Q = N / P;
-    if constexpr (Q == 1) {
+    if constexpr (Q > 1 && ct_gcd(P, Q) == 1) {
+      constexpr int a = ct_modinv(Q % P, P), b = ct_modinv(P % Q, Q);
+      static_assert(a > 0 && b > 0, "prime-factor mapping needs coprime factors");
+      float2 y[P][Q];
+      sfor<0, Q>([&](auto n2c) {
+        constexpr int n2 = decltype(n2c)::value;
+        float2 t[P];
+        sfor<0, P>([&](auto n1) { t[decltype(n1)::value] = x[(Q * decltype(n1)::value + P * n2) % N]; });
+        Dft<P, INV>::run(t);
+        sfor<0, P>([&](auto k1c) { y[decltype(k1c)::value][n2] = t[decltype(k1c)::value]; });
+      });
+      sfor<0, P>([&](auto k1c) {
+        constexpr int k1 = decltype(k1c)::value;
+        Dft<Q, INV>::run(y[k1]);
+        sfor<0, Q>([&](auto k2c) {
+          constexpr int k2 = decltype(k2c)::value;
+          x[(Q * a * k1 + P * b * k2) % N] = y[k1][k2];
+        });
+      });
+    } else if constexpr (Q == 1) {
       float2 y[N];
       sfor<0, N>([&](auto k) {
         constexpr int K = decltype(k)::value;

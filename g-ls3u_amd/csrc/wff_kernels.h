@@ -147,7 +147,8 @@ __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_kernel(
 // D col = IFFT_y(acc).  `ttab` (transition twiddles in LDS) is filled by the
 // caller before the first item.  HOLD: keep each wy spectrum in registers
 // across both of its products, else read it again for the second one.
-template <class P, bool HOLD = true>
+// ZSKIP: how a skipped transform rejoins the loop (below).
+template <class P, bool HOLD = true, bool ZSKIP = true>
 __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* __restrict__ D,
                                           const float* __restrict__ Hy,
                                           const float2* __restrict__ tw_g, float thr, int W,
@@ -199,19 +200,27 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
     }
     // a column whose thresholded signal is zero everywhere adds exactly
     // nothing: skip its transform when no slot of the wavefront has a
-    // surviving sample (wave-uniform branch)
+    // surviving sample (wave-uniform branch).  ZSKIP: both branches define the
+    // same registers (zeros here) and the sum below runs on both, adding exact
+    // zeros after a skip; with a `continue`, the two paths into the loop latch
+    // hold the running sum in different registers and every iteration pays 2E
+    // register copies.  In the persistent batched kernel the zero-sum form
+    // costs registers (an occupancy step at E = 20), so it keeps the `continue`.
     if (__builtin_amdgcn_ballot_w64(live) == 0) {
       if (nskip && lane == 0) atomicAdd(nskip, 1u);  // diagnostics: transforms skipped
-      continue;
-    }
-    fft_fwd<P>(v, buf, ctx);
-    if constexpr (!HOLD) {
-      // the opaque copy of the lane offset defeats load CSE with the first read
-      int l2 = lane;
-      __asm__ volatile("" : "+v"(l2));
-      const float* hy2 = Hy + (size_t)q * N + l2;
+      if constexpr (!ZSKIP) continue;
 #pragma unroll
-      for (int j = 0; j < E; ++j) h[j] = hy2[j * T];
+      for (int j = 0; j < E; ++j) v[j] = make_float2(0.f, 0.f);
+    } else {
+      fft_fwd<P>(v, buf, ctx);
+      if constexpr (!HOLD) {
+        // the opaque copy of the lane offset defeats load CSE with the first read
+        int l2 = lane;
+        __asm__ volatile("" : "+v"(l2));
+        const float* hy2 = Hy + (size_t)q * N + l2;
+#pragma unroll
+        for (int j = 0; j < E; ++j) h[j] = hy2[j * T];
+      }
     }
 #pragma unroll
     for (int j = 0; j < E; ++j) {
@@ -444,7 +453,7 @@ __global__ __launch_bounds__(64 * WAVES) void wff_cols_batch_kernel(
     const int lb = b - s.col0;  // plane lb % nwx, columns [(lb / nwx) * ipb, +ipb)
     const int x = (lb / nwx) * (blockDim.x / T) + slot;
     if (x < s.W)
-      cols_body<P, (P::E > 24)>(s.A, s.D, Hy, tw_g, *s.thr, s.W, s.H, nwx, nwy, s.plane, (lb % nwx) * s.W + x,
+      cols_body<P, (P::E > 24), false>(s.A, s.D, Hy, tw_g, *s.thr, s.W, s.H, nwx, nwy, s.plane, (lb % nwx) * s.W + x,
                    opaque_lane<P>(), ttab, buf);
   }
 }

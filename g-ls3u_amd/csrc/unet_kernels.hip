@@ -241,6 +241,234 @@ __global__ __launch_bounds__(256) void conv3_mfma_kernel(
 }
 
 // ------------------------------------------------------------------------
+// 3x3 convolution, the same implicit GEMM, persistent and software-pipelined,
+// with its addressing reduced to per-thread constants (the kernel above spends
+// more issue cycles on address arithmetic than on MFMAs).  Used for 32 output
+// channels (the full-resolution layers); with 64-channel tiles the per-item
+// restaging of 9 x 64 x 32 weights made it slower than the kernel above.
+// Workgroup: 4 waves in a 2 x 2 grid, output tile 8 rows x 32 PX pixels x NCO
+// channels (NCO = 32: PX = 2); wave (wr, wc) owns rows
+// 4 wr .. 4 wr + 3 and pixels 16 PX wc .. + 16 PX - 1.  The workgroup runs the
+// items (tile, 32-channel chunk) of tiles b, b + G, ... (G workgroups); the
+// next item's input tile (10 x (32 PX + 2) pixels) and weights (9 x NCO x 32,
+// loaded once when there is a single chunk) are read into registers while the
+// current item's MFMAs run and stored to LDS after them, two barriers per
+// item.  LDS rows are 32 PX + 8 pixels apart (a multiple of 8), so the bank
+// swizzle of a fragment depends only on the lane and dx and every fragment
+// read is an immediate offset from one of three per-lane bases; staging slots
+// and their global offsets are per-thread constants, and tiles away from the
+// image border load without clamping.  The taps run by column dx: each of the
+// wave's 6 input rows is read once per dx (PX 16-pixel fragments) and applied
+// to every output row that uses it (dy = input row - output row).
+// Same modes and results (up to the FP32 summation order) as the kernel above.
+constexpr int kRTileH = 8, kRInRows = kRTileH + 2;
+template <int NCO>
+struct RowsConv {
+  static constexpr int PX = NCO == 32 ? 2 : 1;       // 16-pixel blocks per wave row
+  static constexpr int Q = NCO / 16;                 // 16-channel blocks
+  static constexpr int TW = 32 * PX;                 // tile width (pixels)
+  static constexpr int InW = TW + 2;                 // staged pixels per row
+  static constexpr int Pitch = TW + 8;               // LDS pixels per tile row
+  static constexpr int Slots = kRInRows * Pitch * 4; // 16-byte LDS slots of the tile
+  static constexpr int Ld = (Slots + 255) / 256;     // per thread
+  static constexpr int WSlots = 9 * NCO * 4;         // weight slots (taps x co x groups)
+  static constexpr int Wl = (WSlots + 255) / 256;
+};
+
+// byte offset of 16-byte group g of LDS pixel p (bank swizzle as swz)
+__device__ __forceinline__ int swz_b(int p, int g) { return p * 64 + ((g ^ ((p >> 1) & 3)) << 4); }
+
+template <int NCO, int MODE>
+__global__ __launch_bounds__(256, 2) void conv3_rows_kernel(
+    const __bf16* __restrict__ X1, int C1, const __bf16* __restrict__ X2, int C2,
+    const __bf16* __restrict__ Wt, const float* __restrict__ bias,
+    const __bf16* __restrict__ mask, __bf16* __restrict__ Y, int n, int H, int W, int Cout) {
+  static_assert(NCO == 32, "64-channel tiles use conv3_mfma_kernel");
+  using K = RowsConv<NCO>;
+  constexpr int Q = K::Q, P = K::PX, R = 4;
+  constexpr int kRTileW = K::TW, kRInW = K::InW, kRPitch = K::Pitch;
+  constexpr int kRLd = K::Ld, kRWl = K::Wl;
+  __shared__ __attribute__((aligned(16))) char sIn[kRInRows * kRPitch * 64];
+  __shared__ __attribute__((aligned(16))) char sW[K::WSlots * 16];
+  const int Cin = C1 + C2;
+  const bool one_chunk = Cin == 32;
+  const int tilesx = (W + kRTileW - 1) / kRTileW;
+  const int tilesy = H / kRTileH;
+  const int ntiles = n * tilesy * tilesx;
+  const int co0 = blockIdx.y * NCO;
+  const int tid = threadIdx.x;
+  const int wave = tid / 64, lane = tid % 64;
+  const int wr = wave / 2, wc = wave % 2;
+  const int lr = lane % 16, lk = lane / 16;
+
+  // staging slot r of this thread: LDS pixel sp = tid/4 + 64 r, group sg =
+  // tid % 4; its LDS byte offset is sbase + 4096 r (the swizzle term of sp is
+  // (tid >> 3) & 3 for every r)
+  const int sg = tid & 3;
+  const int sbase = swz_b(tid >> 2, sg);
+  unsigned live = 0;   // bit r: slot r is a tile pixel (not row padding / past the end)
+  int off1[kRLd];      // X1 element offset from the item's corner (interior tiles)
+#pragma unroll
+  for (int r = 0; r < kRLd; ++r) {
+    const int sp = (tid >> 2) + 64 * r, ry = sp / kRPitch, rx = sp % kRPitch;
+    const bool ok = sp < kRInRows * kRPitch && rx < kRInW;
+    live |= (ok ? 1u : 0u) << r;
+    off1[r] = (ry * W + rx) * C1 + sg * 8;
+  }
+  // weight slot r: (tap t, channel co) = sp / 32, sp % 32 at the same LDS offsets
+  int offw[kRWl];
+  unsigned wlive = 0;
+#pragma unroll
+  for (int r = 0; r < kRWl; ++r) {
+    const int sp = (tid >> 2) + 64 * r, t = sp / NCO, co = sp % NCO;
+    const bool ok = sp < 9 * NCO && co0 + co < Cout;
+    wlive |= (ok ? 1u : 0u) << r;
+    offw[r] = ((min(co0 + co, Cout - 1) * 9 + min(t, 8)) * Cin) + sg * 8;
+  }
+
+  bf16x8 ld[kRLd], wl[kRWl];
+  auto load_w = [&](int ci0) {
+#pragma unroll
+    for (int r = 0; r < kRWl; ++r) {
+      const bf16x8 v = *(const bf16x8*)(Wt + offw[r] + ci0);
+      wl[r] = (wlive >> r) & 1 ? v : zero8();
+    }
+  };
+  auto load_item = [&](int tile, int ci0, bool with_w) {
+    const int img = tile / (tilesx * tilesy);
+    const int y0 = ((tile / tilesx) % tilesy) * kRTileH, x0 = (tile % tilesx) * kRTileW;
+    const bool interior = y0 >= 1 && y0 + kRTileH + 1 <= H && x0 >= 1 && x0 + kRTileW + 1 <= W;
+    if (interior && ci0 < C1) {
+      const __bf16* base = X1 + (((size_t)img * H + (y0 - 1)) * W + (x0 - 1)) * C1 + ci0;
+#pragma unroll
+      for (int r = 0; r < kRLd; ++r) ld[r] = *(const bf16x8*)(base + off1[r]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < kRLd; ++r) {
+        const int sp = min((tid >> 2) + 64 * r, kRInRows * kRPitch - 1);
+        const int ry = sp / kRPitch, rx = min(sp % kRPitch, kRInW - 1);
+        ld[r] = load8_clamped(X1, C1, X2, C2, img, y0 + ry - 1, x0 + rx - 1, H, W, ci0 + sg * 8);
+      }
+    }
+    if (with_w) load_w(ci0);
+  };
+  auto store_item = [&](bool with_w) {
+#pragma unroll
+    for (int r = 0; r < kRLd; ++r)
+      if ((live >> r) & 1) *(bf16x8*)(sIn + sbase + 4096 * r) = ld[r];
+    if (with_w) {
+#pragma unroll
+      for (int r = 0; r < kRWl; ++r)
+        if ((tid >> 2) + 64 * r < 9 * NCO) *(bf16x8*)(sW + sbase + 4096 * r) = wl[r];
+    }
+  };
+
+  // fragment bases: pixel (4 wr + r, 16 P wc + 16 p + lr + dx) of the tile sits at
+  // pbase[dx] + (r * kRPitch + 16 p) * 64; weight (tap t, 16 q + lr) at
+  // wbase + (t * NCO + 16 q) * 64
+  int pbase[3];
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx)
+    pbase[dx] = swz_b(4 * wr * kRPitch + 16 * P * wc + lr + dx, lk);
+  const int wbase = swz_b(lr, lk);
+
+  int tile = blockIdx.x;  // uniform per workgroup: every barrier below is reached by all waves
+  if (tile >= ntiles) return;
+  load_item(tile, 0, true);
+  bool first = true;
+  for (; tile < ntiles; tile += gridDim.x) {
+    floatx4 acc[R][P][Q];
+#pragma unroll
+    for (int o = 0; o < R; ++o)
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[o][p][q] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
+      const bool w_now = first || !one_chunk;
+      __syncthreads();  // the previous item's LDS reads are done
+      store_item(w_now);
+      __syncthreads();
+      first = false;
+      {  // the next item's loads, in flight during this item's MFMAs
+        int nt = tile, nc = ci0 + 32;
+        if (nc >= Cin) {
+          nt = tile + gridDim.x;
+          nc = 0;
+        }
+        if (nt < ntiles) load_item(nt, nc, !one_chunk);
+      }
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) {
+        // weights as the A operand, pixels as B (lane: 4 channels of one pixel)
+        bf16x8 wf[3][Q];
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+          for (int q = 0; q < Q; ++q)
+            wf[dy][q] = *(const bf16x8*)(sW + wbase + ((3 * dy + dx) * NCO + 16 * q) * 64);
+#pragma unroll
+        for (int r = 0; r < R + 2; ++r) {  // input row 4 wr + r of the tile
+          bf16x8 pf[P];
+#pragma unroll
+          for (int p = 0; p < P; ++p)
+            pf[p] = *(const bf16x8*)(sIn + pbase[dx] + (r * kRPitch + 16 * p) * 64);
+#pragma unroll
+          for (int o = 0; o < R; ++o) {
+            const int dy = r - o;
+            if (dy < 0 || dy > 2) continue;
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+#pragma unroll
+              for (int q = 0; q < Q; ++q)
+                acc[o][p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[dy][q], pf[p],
+                                                                       acc[o][p][q], 0, 0, 0);
+          }
+        }
+      }
+    }
+    // epilogue: lane holds channels 4 lk .. 4 lk + 3 of block q, pixel lr of block p
+    const int img = tile / (tilesx * tilesy);
+    const int y0 = ((tile / tilesx) % tilesy) * kRTileH, x0 = (tile % tilesx) * kRTileW;
+    const bool full = x0 + kRTileW <= W && co0 + NCO <= Cout;
+    const size_t tbase = (((size_t)img * H + y0 + 4 * wr) * W + x0 + 16 * P * wc) * Cout + co0;
+    float bv[Q][4];  // read per tile: kept out of the registers of the MFMA loop
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        bv[q][j] = MODE == 0 ? bias[min(co0 + q * 16 + 4 * lk + j, Cout - 1)] : 0.f;
+#pragma unroll
+    for (int o = 0; o < R; ++o) {
+      __bf16* yrow = Y + tbase + (size_t)o * W * Cout;
+      const __bf16* mrow = MODE == 1 && mask ? mask + tbase + (size_t)o * W * Cout : nullptr;
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          const int lo = (16 * p + lr) * Cout + 16 * q + 4 * lk;
+          if (!full && (x0 + 16 * P * wc + 16 * p + lr >= W || co0 + 16 * q + 4 * lk >= Cout))
+            continue;
+          bf16x4 out;
+          bf16x4 mk;
+          if (MODE == 1 && mask) mk = *(const bf16x4*)(mrow + lo);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            float v = acc[o][p][q][j];
+            if (MODE == 0) {
+              v = fmaxf(v + bv[q][j], 0.f);
+            } else if (mask) {
+              v = ((float)mk[j] > 0.f) ? v : 0.f;
+            }
+            out[j] = (__bf16)v;
+          }
+          *(bf16x4*)(yrow + lo) = out;
+        }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
 // Weight gradient of a 3x3 convolution: dW[co][t][ci] += sum_p dY[p][co] *
 // X[p + off_t][ci], one tap per workgroup (blockIdx.z), a 64 x 64 (co, ci)
 // tile (blockIdx.y) and a range of pixels (blockIdx.x, split K; partial sums
@@ -609,9 +837,29 @@ hipError_t unet_conv3(const __bf16* X1, int C1, const __bf16* X2, int C2, const 
                       int Cout, int mode, hipStream_t s) {
   if ((C1 % 32) || (C2 % 32) || (Cout % 16) || (H % kTileH) || (C2 && ((H | W) & 1)))
     return hipErrorInvalidValue;
-  const int tiles = n * (H / kTileH) * ((W + kTileW - 1) / kTileW);
   const int nco = Cout >= 64 ? 64 : 32;
-  dim3 grid(tiles, (Cout + nco - 1) / nco);
+  const int cob = (Cout + nco - 1) / nco;
+  if (nco == 32 && H % kRTileH == 0) {
+    // persistent 8-row kernel: two workgroups per CU (64 KB LDS, <= 256 registers)
+    static int cus = 0;
+    if (cus == 0) {
+      int dev = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return hipErrorInvalidValue;
+    }
+    const int rtiles = n * (H / kRTileH) * ((W + RowsConv<32>::TW - 1) / RowsConv<32>::TW);
+    dim3 grid(std::min(rtiles, std::max(1, 2 * cus / cob)), cob);
+    if (mode == 0)
+      hipLaunchKernelGGL((conv3_rows_kernel<32, 0>), grid, dim3(256), 0, s, X1, C1, X2, C2, Wt,
+                         bias, mask, Y, n, H, W, Cout);
+    else
+      hipLaunchKernelGGL((conv3_rows_kernel<32, 1>), grid, dim3(256), 0, s, X1, C1, X2, C2, Wt,
+                         bias, mask, Y, n, H, W, Cout);
+    return hipGetLastError();
+  }
+  const int tiles = n * (H / kTileH) * ((W + kTileW - 1) / kTileW);
+  dim3 grid(tiles, cob);
   if (nco == 64) {
     if (mode == 0)
       hipLaunchKernelGGL((conv3_mfma_kernel<64, 0>), grid, dim3(256), 0, s, X1, C1, X2, C2, Wt,

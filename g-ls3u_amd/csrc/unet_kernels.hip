@@ -73,10 +73,18 @@ __device__ __forceinline__ bf16x8 load8_clamped(const __bf16* __restrict__ X1, i
 // MODE 0: y = relu(acc + bias).  MODE 1: y = acc * (mask > 0) (mask may be
 // null: y = acc).  Used for the forward pass (weights [Cout][9][Cin]) and
 // for the input gradient (dY as input, flipped transposed weights).
+// LDS rows of the staged tile are kInPitch = 72 pixels apart (a multiple of
+// 8), so the bank swizzle of a fragment depends only on the lane and dx: the
+// fragment reads are immediate offsets from three per-lane bases, and the
+// staging slots, their LDS offsets and (away from the image border) their
+// global offsets are per-thread constants.
 constexpr int kTileW = 64, kTileH = 4;
-constexpr int kInPix = (kTileH + 2) * (kTileW + 2);
+constexpr int kInW = kTileW + 2, kInPitch = kTileW + 8;
+constexpr int kInSlots = (kTileH + 2) * kInPitch * 4;  // 16-byte slots of the staged tile
 
 __device__ __forceinline__ int swz(int p, int g) { return p * 32 + ((g ^ ((p >> 1) & 3)) << 3); }
+// byte offset of 16-byte group g of LDS pixel p (the same swizzle)
+__device__ __forceinline__ int swz_b(int p, int g) { return p * 64 + ((g ^ ((p >> 1) & 3)) << 4); }
 
 template <int NCO, int MODE>
 __global__ __launch_bounds__(256) void conv3_mfma_kernel(
@@ -88,8 +96,8 @@ __global__ __launch_bounds__(256) void conv3_mfma_kernel(
   // in LDS, one row of 3 taps at a time; 32-channel tiles read them from
   // global memory (L1), which leaves LDS for more resident workgroups
   constexpr bool kWLds = NCO >= 64;
-  __shared__ __attribute__((aligned(16))) __bf16 sIn[kInPix * 32];
-  __shared__ __attribute__((aligned(16))) __bf16 sW[kWLds ? 3 * NCO * 32 : 8];
+  __shared__ __attribute__((aligned(16))) char sIn[kInSlots * 16];
+  __shared__ __attribute__((aligned(16))) char sW[kWLds ? 3 * NCO * 64 : 16];
   const int Cin = C1 + C2;
   const int tilesx = (W + kTileW - 1) / kTileW;
   const int tilesy = H / kTileH;
@@ -98,8 +106,30 @@ __global__ __launch_bounds__(256) void conv3_mfma_kernel(
   const int ty = (tile / tilesx) % tilesy, tx = tile % tilesx;
   const int y0 = ty * kTileH, x0 = tx * kTileW;
   const int co0 = blockIdx.y * NCO;
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int tid = threadIdx.x;
+  const int wave = tid / 64, lane = tid % 64;
   const int lr = lane % 16, lk = lane / 16;
+
+  // staging slot r of this thread: LDS pixel sp = tid/4 + 64 r, group sg =
+  // tid % 4, at byte sbase + 4096 r (the swizzle term of sp is (tid >> 3) & 3
+  // for every r); live bit r: the slot is a tile pixel, not row padding
+  constexpr int kLd = (kInSlots + 255) / 256;
+  const int sg = tid & 3;
+  const int sbase = swz_b(tid >> 2, sg);
+  const bool interior = y0 >= 1 && y0 + kTileH + 1 <= H && x0 >= 1 && x0 + kTileW + 1 <= W;
+  unsigned live = 0;
+  int off1[kLd];  // X1 element offset from the tile's corner (interior tiles)
+#pragma unroll
+  for (int r = 0; r < kLd; ++r) {
+    const int sp = (tid >> 2) + 64 * r, ry = sp / kInPitch, rx = sp % kInPitch;
+    live |= (sp < kInSlots / 4 && rx < kInW ? 1u : 0u) << r;
+    off1[r] = (ry * W + rx) * C1 + sg * 8;
+  }
+  // fragment bases: pixel (wave + dy, 16 mb + lr + dx) of the tile sits at
+  // pbase[dx] + (dy * kInPitch + 16 mb) * 64
+  int pbase[3];
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx) pbase[dx] = swz_b(wave * kInPitch + lr + dx, lk);
 
   floatx4 acc[4][NB];
 #pragma unroll
@@ -122,7 +152,7 @@ __global__ __launch_bounds__(256) void conv3_mfma_kernel(
     bf16x8 a[4];
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb)
-      a[mb] = *(const bf16x8*)(sIn + swz((wave + dy) * (kTileW + 2) + mb * 16 + lr + dx, lk));
+      a[mb] = *(const bf16x8*)(sIn + pbase[dx] + (dy * kInPitch + mb * 16) * 64);
     // weights as the A operand, pixels as B: the 16 x 16 result is
     // [output channel][pixel], so each lane ends with 4 consecutive channels
     // of one pixel (one 8-byte NHWC store in the epilogue)
@@ -136,25 +166,32 @@ __global__ __launch_bounds__(256) void conv3_mfma_kernel(
   // weight staging (NCO >= 64): one row of three taps, 3 * NCO * 32 bf16, as
   // kWl 16-byte loads per thread, all issued before any is stored; the next
   // row's loads are in flight during the current row's MFMAs
+  // weight slot r (tap tl of the row, channel co = sp / NCO, sp % NCO) at the
+  // same LDS offsets as the tile's slots; its global offset less 3 tg Cin + ci0
   constexpr int kWl = (3 * NCO * 4 + 255) / 256;
   bf16x8 wl[kWLds ? kWl : 1];
+  int offw[kWLds ? kWl : 1];
+  unsigned wlive = 0;
+  if constexpr (kWLds) {
+#pragma unroll
+    for (int r = 0; r < kWl; ++r) {
+      const int sp = (tid >> 2) + 64 * r, tl = min(sp / NCO, 2), co = sp % NCO;
+      wlive |= (sp < 3 * NCO && co0 + co < Cout ? 1u : 0u) << r;
+      offw[r] = (min(co0 + co, Cout - 1) * 9 + tl) * Cin + sg * 8;
+    }
+  }
+  const int wbase = swz_b(lr, lk);
   auto load_w = [&](int tg, int ci0) {
 #pragma unroll
     for (int r = 0; r < kWl; ++r) {
-      const int i = min((int)threadIdx.x + 256 * r, 3 * NCO * 4 - 1);
-      const int g = i & 3, rr = i >> 2;
-      const int tl = rr / NCO, co = rr % NCO;
-      const int coc = min(co0 + co, Cout - 1);
-      const bf16x8 v = *(const bf16x8*)(Wt + ((size_t)coc * 9 + 3 * tg + tl) * Cin + ci0 + g * 8);
-      wl[r] = co0 + co < Cout ? v : zero8();
+      const bf16x8 v = *(const bf16x8*)(Wt + offw[r] + 3 * tg * Cin + ci0);
+      wl[r] = (wlive >> r) & 1 ? v : zero8();
     }
   };
   auto store_w = [&]() {
 #pragma unroll
-    for (int r = 0; r < kWl; ++r) {
-      const int i = (int)threadIdx.x + 256 * r;
-      if (i < 3 * NCO * 4) *(bf16x8*)(sW + swz(i >> 2, i & 3)) = wl[r];
-    }
+    for (int r = 0; r < kWl; ++r)
+      if ((tid >> 2) + 64 * r < 3 * NCO) *(bf16x8*)(sW + sbase + 4096 * r) = wl[r];
   };
 
   for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
@@ -163,33 +200,35 @@ __global__ __launch_bounds__(256) void conv3_mfma_kernel(
     if constexpr (kWLds) load_w(0, ci0);
     // the chunk's input tile: every 16-byte load of this thread is issued
     // before the first is stored (one memory round trip, not one per load)
-    constexpr int kLd = (kInPix * 4 + 255) / 256;
     bf16x8 ld[kLd];
+    if (interior && ci0 < C1) {
+      const __bf16* base = X1 + (((size_t)img * H + (y0 - 1)) * W + (x0 - 1)) * C1 + ci0;
 #pragma unroll
-    for (int r = 0; r < kLd; ++r) {
-      const int i = min((int)threadIdx.x + 256 * r, kInPix * 4 - 1);
-      const int g = i & 3, pix = i >> 2;
-      const int ry = pix / (kTileW + 2), rx = pix % (kTileW + 2);
-      ld[r] = load8_clamped(X1, C1, X2, C2, img, y0 + ry - 1, x0 + rx - 1, H, W, ci0 + g * 8);
+      for (int r = 0; r < kLd; ++r) ld[r] = *(const bf16x8*)(base + off1[r]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < kLd; ++r) {
+        const int sp = min((tid >> 2) + 64 * r, kInSlots / 4 - 1);
+        const int ry = sp / kInPitch, rx = min(sp % kInPitch, kInW - 1);
+        ld[r] = load8_clamped(X1, C1, X2, C2, img, y0 + ry - 1, x0 + rx - 1, H, W, ci0 + sg * 8);
+      }
     }
     __syncthreads();  // previous chunk fully consumed
 #pragma unroll
-    for (int r = 0; r < kLd; ++r) {
-      const int i = (int)threadIdx.x + 256 * r;
-      if (i < kInPix * 4) *(bf16x8*)(sIn + swz(i >> 2, i & 3)) = ld[r];
-    }
+    for (int r = 0; r < kLd; ++r)
+      if ((live >> r) & 1) *(bf16x8*)(sIn + sbase + 4096 * r) = ld[r];
     if constexpr (kWLds) {
-#pragma unroll 1
+#pragma unroll
       for (int tg = 0; tg < 3; ++tg) {
         if (tg > 0) __syncthreads();  // previous tap row consumed
         store_w();
         if (tg < 2) load_w(tg + 1, ci0);
         __syncthreads();
-#pragma unroll 1
+#pragma unroll
         for (int tl = 0; tl < 3; ++tl) {
 #pragma unroll
           for (int nb = 0; nb < NB; ++nb)
-            b[nb] = *(const bf16x8*)(sW + swz(tl * NCO + nb * 16 + lr, lk));
+            b[nb] = *(const bf16x8*)(sW + wbase + (tl * NCO + nb * 16) * 64);
           tap(3 * tg + tl, b);
         }
       }
@@ -274,9 +313,6 @@ struct RowsConv {
   static constexpr int WSlots = 9 * NCO * 4;         // weight slots (taps x co x groups)
   static constexpr int Wl = (WSlots + 255) / 256;
 };
-
-// byte offset of 16-byte group g of LDS pixel p (bank swizzle as swz)
-__device__ __forceinline__ int swz_b(int p, int g) { return p * 64 + ((g ^ ((p >> 1) & 3)) << 4); }
 
 template <int NCO, int MODE>
 __global__ __launch_bounds__(256, 2) void conv3_rows_kernel(

@@ -29,33 +29,48 @@ __device__ __forceinline__ void dpra_ls_body(const InT* __restrict__ I,
   __shared__ float sC[TY + 2][TX + 2];  // cos(phi/2)
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int x0 = bx * TX - 1, y0 = by * TY - 1;
-  // Staging of the (TX+2) x (TY+2) tile with its halo: KS slots per thread.
-  // Every load is issued before any is used, from clamped (always in-bounds)
-  // coordinates, and the out-of-image values are replaced afterwards: loads
-  // under per-slot guards would each wait for the previous slot's round trip.
-  constexpr int NS = (TX + 2) * (TY + 2), KS = (NS + NT - 1) / NT;
-  float ld_i[KS], ld_p[KS];
-  bool in[KS];
+  // Staging of the (TX+2) x (TY+2) tile with its halo: every thread stages
+  // its own pixel (LDS (ty+1, tx+1)) and the first NH threads one halo pixel
+  // each (top row, bottom row, left column, right column), so the slot
+  // coordinates need no division.  Every load is issued before any is used,
+  // from clamped (always in-bounds) coordinates, and the out-of-image values
+  // are replaced afterwards: loads under per-slot guards would each wait for
+  // the previous slot's round trip.
+  constexpr int NH = 2 * (TX + 2) + 2 * TY;
+  static_assert(NH <= NT, "at most one halo pixel per thread");
+  const int h = threadIdx.x;
+  const bool halo = h < NH;
+  int hy, hx;  // halo slot of thread h (h < NH)
+  if (h < TX + 2) {
+    hy = 0; hx = h;
+  } else if (h < 2 * (TX + 2)) {
+    hy = TY + 1; hx = h - (TX + 2);
+  } else if (h < 2 * (TX + 2) + TY) {
+    hy = h - 2 * (TX + 2) + 1; hx = 0;
+  } else {
+    hy = min(h - 2 * (TX + 2) - TY + 1, TY); hx = TX + 1;
+  }
+  const int ly[2] = {ty + 1, hy}, lx[2] = {tx + 1, hx};
+  float ld_i[2], ld_p[2];
+  bool in[2];
 #pragma unroll
-  for (int k = 0; k < KS; ++k) {
-    const int i = min((int)threadIdx.x + k * NT, NS - 1);
-    const int ly = i / (TX + 2), lx = i % (TX + 2);
-    const int gx = x0 + lx, gy = y0 + ly;
+  for (int k = 0; k < 2; ++k) {
+    const int gx = x0 + lx[k], gy = y0 + ly[k];
     in[k] = gx >= 0 && gx < W && gy >= 0 && gy < H;
     const int cx = min(max(gx, 0), W - 1), cy = min(max(gy, 0), H - 1);
-    ld_i[k] = load_in(I, (size_t)cy * pitch + cx);
-    ld_p[k] = phr[(size_t)cy * W + cx];
+    if (k == 0 || halo) {
+      ld_i[k] = load_in(I, (size_t)cy * pitch + cx);
+      ld_p[k] = phr[(size_t)cy * W + cx];
+    }
   }
 #pragma unroll
-  for (int k = 0; k < KS; ++k) {
-    const int i = (int)threadIdx.x + k * NT;
-    if (i >= NS) break;
-    const int ly = i / (TX + 2), lx = i % (TX + 2);
+  for (int k = 0; k < 2; ++k) {
+    if (k == 1 && !halo) break;
     float hs = 0.f, hc = 1.f;
     sincos_half(0.5f * ld_p[k], &hs, &hc);
-    sI[ly][lx] = in[k] ? ld_i[k] : 0.f;
-    sS[ly][lx] = in[k] ? hs : 0.f;
-    sC[ly][lx] = in[k] ? hc : 1.f;
+    sI[ly[k]][lx[k]] = in[k] ? ld_i[k] : 0.f;
+    sS[ly[k]][lx[k]] = in[k] ? hs : 0.f;
+    sC[ly[k]][lx[k]] = in[k] ? hc : 1.f;
   }
   __syncthreads();
   const int x = bx * TX + tx, y = by * TY + ty;

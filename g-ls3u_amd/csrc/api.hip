@@ -1086,7 +1086,15 @@ extern "C" int glsu_dpra_batch_create(glsu_dpra_batch** out, const glsu_seq_desc
     pr = &def;
   }
   // host-side validation of every sequence and its plan choice
-  struct Req { int px, py, cw, rw; };
+  // lx, ly: whether the image fits the live registers of its row / column
+  // plan (the kernels compiled for them round slightly differently, and a
+  // class shares one choice)
+  struct Req { int px, py, cw, rw, lx, ly; };
+  auto live_fit = [](int plan, int n) {
+    for (const FftPlanInfo& q : fft_plans())
+      if (q.id == plan) return n <= live_regs(q.E) * q.T ? 1 : 0;
+    return 0;
+  };
   std::vector<Req> req(n);
   for (int i = 0; i < n; ++i) {
     const glsu_seq_desc& d = seqs[i];
@@ -1096,7 +1104,8 @@ extern "C" int glsu_dpra_batch_create(glsu_dpra_batch** out, const glsu_seq_desc
     int px = t ? t->planx : -1, py = t ? t->plany : -1;
     if (int rc = validate(d.width, d.height, pr, &px, &py)) return rc;
     if (int rc = validate_tuning(t)) return rc;
-    req[i] = {px, py, t ? t->cols_waves : -1, t ? t->rows_waves : -1};
+    req[i] = {px, py, t ? t->cols_waves : -1, t ? t->rows_waves : -1, live_fit(px, d.width),
+              live_fit(py, d.height)};
   }
   if (int rc = check_device()) return rc;
   auto* h = new glsu_dpra_batch();
@@ -1114,7 +1123,7 @@ extern "C" int glsu_dpra_batch_create(glsu_dpra_batch** out, const glsu_seq_desc
     int c = -1;
     for (size_t k = 0; k < keys.size(); ++k)
       if (keys[k].px == req[i].px && keys[k].py == req[i].py && keys[k].cw == req[i].cw &&
-          keys[k].rw == req[i].rw)
+          keys[k].rw == req[i].rw && keys[k].lx == req[i].lx && keys[k].ly == req[i].ly)
         c = (int)k;
     if (c < 0) {
       c = (int)keys.size();
@@ -1156,10 +1165,12 @@ extern "C" int glsu_dpra_batch_create(glsu_dpra_batch** out, const glsu_seq_desc
   for (auto& c : h->cls) {
     const WffPlan& p = c.p;
     const int ipb = cols_items_per_block(p.plany, p.cols_waves);
-    int row = 0, blk = 0;
+    int row = 0, blk = 0, maxW = 0, maxH = 0;
     for (int k = c.b; k < c.e; ++k) {
       BatchSeg& sg = h->h_tab[k];
       const BatchImage& im = h->img[k];
+      maxW = std::max(maxW, im.W);
+      maxH = std::max(maxH, im.H);
       sg = BatchSeg{};
       sg.W = im.W; sg.H = im.H; sg.rr = im.rr;
       sg.row0 = row; sg.col0 = blk; sg.tile0 = tile;
@@ -1176,6 +1187,7 @@ extern "C" int glsu_dpra_batch_create(glsu_dpra_batch** out, const glsu_seq_desc
     b.planx = p.planx; b.plany = p.plany;
     b.cols_waves = p.cols_waves; b.rows_waves = p.rows_waves;
     b.nwx = p.nwx; b.nwy = p.nwy; b.scale = p.scale;
+    b.maxW = maxW; b.maxH = maxH;
     b.Hx = p.d_Hx; b.Hy = p.d_Hy; b.twx = p.d_twx; b.twy = p.d_twy;
     b.grid_rows = batch_grid_rows(p.planx, p.rows_waves, row);
     b.grid_cols = batch_grid_cols(p.plany, p.cols_waves, blk);

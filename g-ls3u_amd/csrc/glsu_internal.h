@@ -188,6 +188,11 @@ struct BatchSeg {
 };
 
 // Plan-class launch data of the batched WFF2 stages.
+// Registers per lane of a transform of E points per lane that a kernel treats
+// as live when the image extent fits in them (the rest hold only zero inputs
+// and discarded outputs): E - 4 from 16 points per lane up.
+constexpr int live_regs(int E) { return E >= 16 ? E - 4 : E; }
+
 struct BatchClass {
   const BatchSeg* tab;       // device table of this class's images (contiguous)
   int nseg;
@@ -195,6 +200,7 @@ struct BatchClass {
   int ncolblk;               // sum over images of nwx * ceil(W / items per block)
   int planx, plany, cols_waves, rows_waves;
   int nwx, nwy;
+  int maxW, maxH;            // largest image of the class (live-register choice)
   float scale;
   const float *Hx, *Hy;
   const float2 *twx, *twy;

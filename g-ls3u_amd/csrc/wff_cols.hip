@@ -3,31 +3,32 @@
 
 namespace glsu {
 
+template <class P, int WAVES, int EV>
+static void cols_launch(const WffPlan& p, int grid, int smem, hipStream_t s, const int* skip) {
+  hipLaunchKernelGGL((wff_cols_kernel<P, WAVES, EV>), dim3(grid), dim3(64 * WAVES), smem, s,
+                     p.d_A, p.d_D, p.d_Hy, p.d_twy, p.d_thr, p.W, p.H, p.nwx, p.nwy,
+                     p.plane_elems, skip, p.d_nskip);
+}
+template <class P, int WAVES>
+static void cols_launch_live(const WffPlan& p, int grid, int smem, hipStream_t s,
+                             const int* skip) {
+  if (live_regs_fit<P>(p.H)) cols_launch<P, WAVES, kLiveRegs<P>>(p, grid, smem, s, skip);
+  else cols_launch<P, WAVES, P::E>(p, grid, smem, s, skip);
+}
+
 hipError_t launch_cols(const WffPlan& p, hipStream_t s, const int* skip) {
   const int waves = p.cols_waves;
   const int smem = plan_smem_bytes(p.plany, waves, true);
   switch (p.plany) {
-#define BODY                                                                                    \
-  {                                                                                             \
-    const int per = items_per_block<P>(waves);                                                  \
-    const int grid = p.nwx * ((p.W + per - 1) / per);                                           \
-    switch (waves) {                                                                            \
-      case 1:                                                                                   \
-        hipLaunchKernelGGL((wff_cols_kernel<P, 1>), dim3(grid), dim3(64), smem, s, p.d_A,      \
-                           p.d_D, p.d_Hy, p.d_twy, p.d_thr, p.W, p.H, p.nwx, p.nwy,             \
-                           p.plane_elems, skip, p.d_nskip);                                     \
-        break;                                                                                  \
-      case 2:                                                                                   \
-        hipLaunchKernelGGL((wff_cols_kernel<P, 2>), dim3(grid), dim3(128), smem, s, p.d_A,     \
-                           p.d_D, p.d_Hy, p.d_twy, p.d_thr, p.W, p.H, p.nwx, p.nwy,             \
-                           p.plane_elems, skip, p.d_nskip);                                     \
-        break;                                                                                  \
-      default:                                                                                  \
-        hipLaunchKernelGGL((wff_cols_kernel<P, 4>), dim3(grid), dim3(256), smem, s, p.d_A,     \
-                           p.d_D, p.d_Hy, p.d_twy, p.d_thr, p.W, p.H, p.nwx, p.nwy,             \
-                           p.plane_elems, skip, p.d_nskip);                                     \
-        break;                                                                                  \
-    }                                                                                           \
+#define BODY                                                                 \
+  {                                                                          \
+    const int per = items_per_block<P>(waves);                               \
+    const int grid = p.nwx * ((p.W + per - 1) / per);                        \
+    switch (waves) {                                                         \
+      case 1: cols_launch_live<P, 1>(p, grid, smem, s, skip); break;         \
+      case 2: cols_launch_live<P, 2>(p, grid, smem, s, skip); break;         \
+      default: cols_launch_live<P, 4>(p, grid, smem, s, skip); break;        \
+    }                                                                        \
   }
     GLSU_PLANS(GLSU_DISPATCH_CASE)
 #undef BODY
@@ -74,24 +75,26 @@ int batch_grid_cols(int plany, int waves, int nblk) {
   return -1;
 }
 
+template <class P, int WAVES, int EV>
+static void cols_batch_launch(const BatchClass& c, int smem, hipStream_t s) {
+  hipLaunchKernelGGL((wff_cols_batch_kernel<P, WAVES, EV>), dim3(c.grid_cols), dim3(64 * WAVES),
+                     smem, s, c.tab, c.nseg, c.ncolblk, c.Hy, c.twy, c.nwx, c.nwy);
+}
+template <class P, int WAVES>
+static void cols_batch_launch_live(const BatchClass& c, int smem, hipStream_t s) {
+  if (live_regs_fit<P>(c.maxH)) cols_batch_launch<P, WAVES, kLiveRegs<P>>(c, smem, s);
+  else cols_batch_launch<P, WAVES, P::E>(c, smem, s);
+}
+
 hipError_t launch_cols_batch(const BatchClass& c, hipStream_t s) {
   const int waves = c.cols_waves;
   const int smem = plan_smem_bytes(c.plany, waves, true);
   switch (c.plany) {
-#define BODY                                                                                 \
-  switch (waves) {                                                                           \
-    case 1:                                                                                  \
-      hipLaunchKernelGGL((wff_cols_batch_kernel<P, 1>), dim3(c.grid_cols), dim3(64), smem, s, \
-                         c.tab, c.nseg, c.ncolblk, c.Hy, c.twy, c.nwx, c.nwy);               \
-      break;                                                                                 \
-    case 2:                                                                                  \
-      hipLaunchKernelGGL((wff_cols_batch_kernel<P, 2>), dim3(c.grid_cols), dim3(128), smem,  \
-                         s, c.tab, c.nseg, c.ncolblk, c.Hy, c.twy, c.nwx, c.nwy);            \
-      break;                                                                                 \
-    default:                                                                                 \
-      hipLaunchKernelGGL((wff_cols_batch_kernel<P, 4>), dim3(c.grid_cols), dim3(256), smem,  \
-                         s, c.tab, c.nseg, c.ncolblk, c.Hy, c.twy, c.nwx, c.nwy);            \
-      break;                                                                                 \
+#define BODY                                                     \
+  switch (waves) {                                               \
+    case 1: cols_batch_launch_live<P, 1>(c, smem, s); break;     \
+    case 2: cols_batch_launch_live<P, 2>(c, smem, s); break;     \
+    default: cols_batch_launch_live<P, 4>(c, smem, s); break;    \
   }
     GLSU_PLANS(GLSU_DISPATCH_CASE)
 #undef BODY

@@ -55,6 +55,22 @@ inline const PlanRow* plan_row(int id) {
   return nullptr;
 }
 
+// Live registers of a transform: when the image extent n along the axis fits
+// in the first EV registers of every lane (n <= EV * T), the last E - EV
+// registers only ever hold zero inputs and discarded outputs; kernels compiled
+// for that EV treat them as such at compile time, and the compiler drops the
+// arithmetic that feeds or reads only them (the 1280-point plan at 1024
+// pixels: 16 of 20 registers live).  Used by the row-forward and column
+// kernels, single and batched alike (batch classes are split by whether their
+// images fit, so both forms round the same way); the row-inverse kernel
+// measured slower with it and keeps all E.
+template <class P>
+inline constexpr int kLiveRegs = live_regs(P::E);
+template <class P>
+inline bool live_regs_fit(int n) {
+  return n <= kLiveRegs<P> * P::T;
+}
+
 __device__ __forceinline__ size_t tile_index(int y, int x, int H) {
   return ((size_t)(x / kTileCols) * H + y) * kTileCols + (x % kTileCols);
 }
@@ -63,7 +79,7 @@ __device__ __forceinline__ size_t tile_index(int y, int x, int H) {
 // One workgroup per row, RW waves (one transform slot each, T == 64).
 // Wave 0 computes Zr = FFT_x(f row) and shares it through LDS; wave w then
 // produces A_wx = IFFT_x(Zr * Hx_wx) for wx = w, w + RW, ...
-template <class P>
+template <class P, int EV = P::E>
 __device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ frow,
                                               float2* __restrict__ A,
                                               const float* __restrict__ Hx,
@@ -82,7 +98,7 @@ __device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ frow,
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       const int x = lane + T * j;
-      z[j] = x < W ? frow[x] : make_float2(0.f, 0.f);
+      z[j] = j < EV && x < W ? frow[x] : make_float2(0.f, 0.f);
     }
     fft_fwd<P>(z, buf, ctx);
 #pragma unroll
@@ -123,12 +139,12 @@ __device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ frow,
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       const int x = lane + T * j;
-      if (x < W) out[tile_index(y, x, H)] = v[j];
+      if (j < EV && x < W) out[tile_index(y, x, H)] = v[j];
     }
   }
 }
 
-template <class P>
+template <class P, int EV = P::E>
 __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_kernel(
     const float2* __restrict__ f, float2* __restrict__ A, const float* __restrict__ Hx,
     const float2* __restrict__ tw_g, int W, int H, int nwx, size_t plane,
@@ -137,7 +153,7 @@ __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_kernel(
   if (skip && *skip) return;  // device-side decision (learned path): not needed
   const int y = xcd_remap(blockIdx.x, gridDim.x);  // uniform per workgroup
   if (y >= H) return;
-  rows_fwd_body<P>(f + (size_t)y * W, A, Hx, tw_g, W, H, nwx, plane, y, threadIdx.x % P::T,
+  rows_fwd_body<P, EV>(f + (size_t)y * W, A, Hx, tw_g, W, H, nwx, plane, y, threadIdx.x % P::T,
                    smem);
 }
 
@@ -148,7 +164,7 @@ __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_kernel(
 // caller before the first item.  HOLD: keep each wy spectrum in registers
 // across both of its products, else read it again for the second one.
 // ZSKIP: how a skipped transform rejoins the loop (below).
-template <class P, bool HOLD = true, bool ZSKIP = true>
+template <class P, bool HOLD = true, bool ZSKIP = true, int EV = P::E>
 __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* __restrict__ D,
                                           const float* __restrict__ Hy,
                                           const float2* __restrict__ tw_g, float thr, int W,
@@ -166,7 +182,7 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     const int y = lane + T * j;
-    ah[j] = y < H ? colA[tile_index(y, x, H)] : make_float2(0.f, 0.f);
+    ah[j] = j < EV && y < H ? colA[tile_index(y, x, H)] : make_float2(0.f, 0.f);
     acc[j] = make_float2(0.f, 0.f);
   }
   fft_fwd<P>(ah, buf, ctx);
@@ -194,7 +210,7 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
     for (int j = 0; j < E; ++j) {
       const int y = lane + T * j;
       const float m2 = v[j].x * v[j].x + v[j].y * v[j].y;
-      const bool keep = y < H && m2 >= thr2;
+      const bool keep = j < EV && y < H && m2 >= thr2;
       if (!keep) v[j] = make_float2(0.f, 0.f);
       live |= keep;
     }
@@ -233,11 +249,11 @@ __device__ __forceinline__ void cols_body(const float2* __restrict__ A, float2* 
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     const int y = lane + T * j;
-    if (y < H) colD[tile_index(y, x, H)] = acc[j];
+    if (j < EV && y < H) colD[tile_index(y, x, H)] = acc[j];
   }
 }
 
-template <class P, int WAVES>
+template <class P, int WAVES, int EV = P::E>
 __global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
     const float2* __restrict__ A, float2* __restrict__ D, const float* __restrict__ Hy,
     const float2* __restrict__ tw_g, const float* __restrict__ thr_p, int W, int H, int nwx,
@@ -257,7 +273,7 @@ __global__ __launch_bounds__(64 * WAVES) void wff_cols_kernel(
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int x = (b / nwx) * (blockDim.x / T) + slot;
   if (x >= W) return;
-  cols_body<P>(A, D, Hy, tw_g, *thr_p, W, H, nwx, nwy, plane, (b % nwx) * W + x,
+  cols_body<P, true, true, EV>(A, D, Hy, tw_g, *thr_p, W, H, nwx, nwy, plane, (b % nwx) * W + x,
                threadIdx.x % T, ttab, buf, nskip);
 }
 
@@ -422,7 +438,7 @@ __device__ __forceinline__ int seg_find(const BatchSeg* __restrict__ tab, int n,
   return lo;
 }
 
-template <class P>
+template <class P, int EV = P::E>
 __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_batch_kernel(
     const BatchSeg* __restrict__ tab, int nseg, int nrows, const float* __restrict__ Hx,
     const float2* __restrict__ tw_g, int nwx) {
@@ -431,12 +447,12 @@ __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_batch_kernel(
     const int r = xcd_remap(v, nrows);
     const BatchSeg& s = tab[seg_find<&BatchSeg::row0>(tab, nseg, r)];
     __syncthreads();  // LDS of the previous row is free
-    rows_fwd_body<P>(s.z + (size_t)(r - s.row0) * s.W, s.A, Hx, tw_g, s.W, s.H, nwx, s.plane,
+    rows_fwd_body<P, EV>(s.z + (size_t)(r - s.row0) * s.W, s.A, Hx, tw_g, s.W, s.H, nwx, s.plane,
                      r - s.row0, opaque_lane<P>(), smem);
   }
 }
 
-template <class P, int WAVES>
+template <class P, int WAVES, int EV = P::E>
 __global__ __launch_bounds__(64 * WAVES) void wff_cols_batch_kernel(
     const BatchSeg* __restrict__ tab, int nseg, int nblk, const float* __restrict__ Hy,
     const float2* __restrict__ tw_g, int nwx, int nwy) {
@@ -453,8 +469,9 @@ __global__ __launch_bounds__(64 * WAVES) void wff_cols_batch_kernel(
     const int lb = b - s.col0;  // plane lb % nwx, columns [(lb / nwx) * ipb, +ipb)
     const int x = (lb / nwx) * (blockDim.x / T) + slot;
     if (x < s.W)
-      cols_body<P, (P::E > 24), false>(s.A, s.D, Hy, tw_g, *s.thr, s.W, s.H, nwx, nwy, s.plane, (lb % nwx) * s.W + x,
-                   opaque_lane<P>(), ttab, buf);
+      cols_body<P, (P::E > 24), false, EV>(s.A, s.D, Hy, tw_g, *s.thr, s.W, s.H, nwx, nwy,
+                                           s.plane, (lb % nwx) * s.W + x, opaque_lane<P>(),
+                                           ttab, buf);
   }
 }
 

@@ -7,12 +7,17 @@ hipError_t launch_rows_fwd(const WffPlan& p, const float2* f, hipStream_t s, con
   const int rw = p.rows_waves;
   const int smem = plan_smem_bytes(p.planx, rw);
   switch (p.planx) {
-#define BODY                                                                              \
-  if constexpr (P::T == 64) {                                                             \
-    hipLaunchKernelGGL((wff_rows_fwd_kernel<P>), dim3(p.H), dim3(64 * rw), smem, s, f,    \
-                       p.d_A, p.d_Hx, p.d_twx, p.W, p.H, p.nwx, p.plane_elems, skip);     \
-  } else {                                                                                \
-    return hipErrorInvalidValue;                                                          \
+#define BODY                                                                                \
+  if constexpr (P::T == 64) {                                                               \
+    if (live_regs_fit<P>(p.W))                                                              \
+      hipLaunchKernelGGL((wff_rows_fwd_kernel<P, kLiveRegs<P>>), dim3(p.H), dim3(64 * rw),  \
+                         smem, s, f, p.d_A, p.d_Hx, p.d_twx, p.W, p.H, p.nwx,               \
+                         p.plane_elems, skip);                                              \
+    else                                                                                    \
+      hipLaunchKernelGGL((wff_rows_fwd_kernel<P>), dim3(p.H), dim3(64 * rw), smem, s, f,    \
+                         p.d_A, p.d_Hx, p.d_twx, p.W, p.H, p.nwx, p.plane_elems, skip);     \
+  } else {                                                                                  \
+    return hipErrorInvalidValue;                                                            \
   }
     GLSU_PLANS(GLSU_DISPATCH_CASE)
 #undef BODY
@@ -56,12 +61,16 @@ hipError_t launch_rows_fwd_batch(const BatchClass& c, hipStream_t s) {
   const int rw = c.rows_waves;
   const int smem = plan_smem_bytes(c.planx, rw);
   switch (c.planx) {
-#define BODY                                                                                  \
-  if constexpr (P::T == 64) {                                                                 \
-    hipLaunchKernelGGL((wff_rows_fwd_batch_kernel<P>), dim3(c.grid_rows), dim3(64 * rw), smem, \
-                       s, c.tab, c.nseg, c.nrows, c.Hx, c.twx, c.nwx);                        \
-  } else {                                                                                    \
-    return hipErrorInvalidValue;                                                              \
+#define BODY                                                                                 \
+  if constexpr (P::T == 64) {                                                                \
+    if (live_regs_fit<P>(c.maxW))                                                            \
+      hipLaunchKernelGGL((wff_rows_fwd_batch_kernel<P, kLiveRegs<P>>), dim3(c.grid_rows),    \
+                         dim3(64 * rw), smem, s, c.tab, c.nseg, c.nrows, c.Hx, c.twx, c.nwx); \
+    else                                                                                     \
+      hipLaunchKernelGGL((wff_rows_fwd_batch_kernel<P>), dim3(c.grid_rows), dim3(64 * rw),   \
+                         smem, s, c.tab, c.nseg, c.nrows, c.Hx, c.twx, c.nwx);               \
+  } else {                                                                                   \
+    return hipErrorInvalidValue;                                                             \
   }
     GLSU_PLANS(GLSU_DISPATCH_CASE)
 #undef BODY

@@ -505,6 +505,174 @@ __global__ __launch_bounds__(256, 2) void conv3_rows_kernel(
 }
 
 // ------------------------------------------------------------------------
+// 3x3 convolution for 64 output channels and at most 64 input channels, with
+// every weight of the workgroup's 64 output channels resident in LDS for the
+// whole kernel (9 x 64 x Cin bf16, loaded once), so that the per-tile work is
+// the input tile only.  Workgroup: 8 waves, 4 x 2 grid, output tile 16 rows x
+// 32 pixels x 64 channels; wave (wr, wc) owns rows 4 wr .. 4 wr + 3 and pixels
+// 16 wc .. 16 wc + 15.  Persistent over items (tile, 32-channel chunk) like
+// conv3_rows_kernel, the next item's 18 x 34-pixel tile loaded into registers
+// during the current item's MFMAs; LDS rows 40 pixels apart, so fragment reads
+// are immediate offsets from per-lane bases.  One workgroup per CU (120 KB of
+// LDS), two waves per SIMD.  Same modes and results (up to the FP32 summation
+// order) as conv3_mfma_kernel.
+constexpr int kXTileH = 16, kXTileW = 32, kXInRows = kXTileH + 2, kXInW = kXTileW + 2,
+              kXPitch = kXTileW + 8;
+constexpr int kXSlots = kXInRows * kXPitch * 4;  // 16-byte slots of the tile
+constexpr int kXLd = (kXSlots + 511) / 512;       // per thread
+constexpr int kXMaxCin = 64;
+
+template <int MODE>
+__global__ __launch_bounds__(512, 1) void conv3_res64_kernel(
+    const __bf16* __restrict__ X1, int C1, const __bf16* __restrict__ X2, int C2,
+    const __bf16* __restrict__ Wt, const float* __restrict__ bias,
+    const __bf16* __restrict__ mask, __bf16* __restrict__ Y, int n, int H, int W, int Cout) {
+  constexpr int NCO = 64, Q = 4, R = 4;
+  __shared__ __attribute__((aligned(16))) char sIn[kXInRows * kXPitch * 64];
+  __shared__ __attribute__((aligned(16))) char sW[(kXMaxCin / 32) * 9 * NCO * 64];
+  const int Cin = C1 + C2, nch = Cin / 32;
+  const int tilesx = (W + kXTileW - 1) / kXTileW;
+  const int tilesy = H / kXTileH;
+  const int ntiles = n * tilesy * tilesx;
+  const int co0 = blockIdx.y * NCO;
+  const int tid = threadIdx.x;
+  const int wave = tid / 64, lane = tid % 64;
+  const int wr = wave / 2, wc = wave % 2;
+  const int lr = lane % 16, lk = lane / 16;
+  const int sg = tid & 3;
+  // staging slot r of this thread: LDS pixel tid/4 + 128 r, group sg, at byte
+  // sbase + 8192 r (the swizzle term of that pixel is (tid >> 3) & 3 for every r)
+  const int sbase = swz_b(tid >> 2, sg);
+
+  // all weights of the workgroup's channels: LDS pixel (c 9 + t) NCO + co
+  {
+    const int nslot = nch * 9 * NCO * 4;
+    for (int i = tid; i < nslot; i += 512) {
+      const int g = i & 3, px = i >> 2;
+      const int co = px % NCO, ct = px / NCO, c = ct / 9, t = ct % 9;
+      bf16x8 v = zero8();
+      if (co0 + co < Cout) v = *(const bf16x8*)(Wt + ((size_t)(co0 + co) * 9 + t) * Cin + c * 32 + g * 8);
+      *(bf16x8*)(sW + swz_b(px, g)) = v;
+    }
+  }
+
+  unsigned live = 0;
+  int off1[kXLd];
+#pragma unroll
+  for (int r = 0; r < kXLd; ++r) {
+    const int sp = (tid >> 2) + 128 * r, ry = sp / kXPitch, rx = sp % kXPitch;
+    live |= (sp < kXInRows * kXPitch && rx < kXInW ? 1u : 0u) << r;
+    off1[r] = (ry * W + rx) * C1 + sg * 8;
+  }
+  bf16x8 ld[kXLd];
+  auto load_item = [&](int tile, int ci0) {
+    const int img = tile / (tilesx * tilesy);
+    const int y0 = ((tile / tilesx) % tilesy) * kXTileH, x0 = (tile % tilesx) * kXTileW;
+    const bool interior = y0 >= 1 && y0 + kXTileH + 1 <= H && x0 >= 1 && x0 + kXTileW + 1 <= W;
+    if (interior && ci0 < C1) {
+      const __bf16* base = X1 + (((size_t)img * H + (y0 - 1)) * W + (x0 - 1)) * C1 + ci0;
+#pragma unroll
+      for (int r = 0; r < kXLd; ++r) ld[r] = *(const bf16x8*)(base + off1[r]);
+    } else {
+      int t2 = tid;
+      __asm__ volatile("" : "+v"(t2));
+#pragma unroll
+      for (int r = 0; r < kXLd; ++r) {
+        const int sp = min((t2 >> 2) + 128 * r, kXInRows * kXPitch - 1);
+        const int ry = sp / kXPitch, rx = min(sp % kXPitch, kXInW - 1);
+        ld[r] = load8_clamped(X1, C1, X2, C2, img, y0 + ry - 1, x0 + rx - 1, H, W, ci0 + sg * 8);
+      }
+    }
+  };
+
+  int pbase[3];
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx) pbase[dx] = swz_b(4 * wr * kXPitch + 16 * wc + lr + dx, lk);
+  const int wbase = swz_b(lr, lk);
+
+  int tile = blockIdx.x;  // uniform per workgroup: every barrier below is reached by all waves
+  if (tile >= ntiles) return;
+  load_item(tile, 0);
+  for (; tile < ntiles; tile += gridDim.x) {
+    floatx4 acc[R][Q];
+#pragma unroll
+    for (int o = 0; o < R; ++o)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) acc[o][q] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < nch; ++c) {
+      __syncthreads();  // the previous item's LDS reads (and the weight stores) are done
+#pragma unroll
+      for (int r = 0; r < kXLd; ++r)
+        if ((live >> r) & 1) *(bf16x8*)(sIn + sbase + 8192 * r) = ld[r];
+      __syncthreads();
+      {  // the next item's loads, in flight during this item's MFMAs
+        int nt = tile, nc = c + 1;
+        if (nc >= nch) {
+          nt = tile + gridDim.x;
+          nc = 0;
+        }
+        if (nt < ntiles) load_item(nt, nc * 32);
+      }
+      const char* wch = sW + c * 9 * NCO * 64;
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) {
+        bf16x8 wf[3][Q];
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+          for (int q = 0; q < Q; ++q)
+            wf[dy][q] = *(const bf16x8*)(wch + wbase + ((3 * dy + dx) * NCO + 16 * q) * 64);
+#pragma unroll
+        for (int r = 0; r < R + 2; ++r) {
+          const bf16x8 pf = *(const bf16x8*)(sIn + pbase[dx] + r * kXPitch * 64);
+#pragma unroll
+          for (int o = 0; o < R; ++o) {
+            const int dy = r - o;
+            if (dy < 0 || dy > 2) continue;
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+              acc[o][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[dy][q], pf, acc[o][q], 0, 0, 0);
+          }
+        }
+      }
+    }
+    const int img = tile / (tilesx * tilesy);
+    const int y0 = ((tile / tilesx) % tilesy) * kXTileH, x0 = (tile % tilesx) * kXTileW;
+    const int x = x0 + 16 * wc + lr;
+    float bv[Q][4];
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        bv[q][j] = MODE == 0 ? bias[min(co0 + q * 16 + 4 * lk + j, Cout - 1)] : 0.f;
+#pragma unroll
+    for (int o = 0; o < R; ++o) {
+      const int y = y0 + 4 * wr + o;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int cb = co0 + q * 16 + 4 * lk;
+        if (x >= W || cb >= Cout) continue;
+        const size_t off = (((size_t)img * H + y) * W + x) * Cout + cb;
+        bf16x4 out;
+        bf16x4 mk;
+        if (MODE == 1 && mask) mk = *(const bf16x4*)(mask + off);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          float v = acc[o][q][j];
+          if (MODE == 0) {
+            v = fmaxf(v + bv[q][j], 0.f);
+          } else if (mask) {
+            v = ((float)mk[j] > 0.f) ? v : 0.f;
+          }
+          out[j] = (__bf16)v;
+        }
+        *(bf16x4*)(Y + off) = out;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
 // Weight gradient of a 3x3 convolution: dW[co][t][ci] += sum_p dY[p][co] *
 // X[p + off_t][ci], one tap per workgroup (blockIdx.z), a 64 x 64 (co, ci)
 // tile (blockIdx.y) and a range of pixels (blockIdx.x, split K; partial sums
@@ -892,6 +1060,25 @@ hipError_t unet_conv3(const __bf16* X1, int C1, const __bf16* X2, int C2, const 
     else
       hipLaunchKernelGGL((conv3_rows_kernel<32, 1>), grid, dim3(256), 0, s, X1, C1, X2, C2, Wt,
                          bias, mask, Y, n, H, W, Cout);
+    return hipGetLastError();
+  }
+  if (nco == 64 && C1 + C2 <= kXMaxCin && H % kXTileH == 0) {
+    // resident-weight kernel: one 8-wave workgroup per CU
+    static int cus = 0;
+    if (cus == 0) {
+      int dev = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return hipErrorInvalidValue;
+    }
+    const int xt = n * (H / kXTileH) * ((W + kXTileW - 1) / kXTileW);
+    dim3 gridx(std::min(xt, std::max(1, cus / cob)), cob);
+    if (mode == 0)
+      hipLaunchKernelGGL((conv3_res64_kernel<0>), gridx, dim3(512), 0, s, X1, C1, X2, C2, Wt, bias,
+                         mask, Y, n, H, W, Cout);
+    else
+      hipLaunchKernelGGL((conv3_res64_kernel<1>), gridx, dim3(512), 0, s, X1, C1, X2, C2, Wt, bias,
+                         mask, Y, n, H, W, Cout);
     return hipGetLastError();
   }
   const int tiles = n * (H / kTileH) * ((W + kTileW - 1) / kTileW);

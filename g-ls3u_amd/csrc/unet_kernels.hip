@@ -505,31 +505,35 @@ __global__ __launch_bounds__(256, 2) void conv3_rows_kernel(
 }
 
 // ------------------------------------------------------------------------
-// 3x3 convolution for 64 output channels and at most 64 input channels, with
-// every weight of the workgroup's 64 output channels resident in LDS for the
-// whole kernel (9 x 64 x Cin bf16, loaded once), so that the per-tile work is
-// the input tile only.  Workgroup: 8 waves, 4 x 2 grid, output tile 16 rows x
-// 32 pixels x 64 channels; wave (wr, wc) owns rows 4 wr .. 4 wr + 3 and pixels
+// 3x3 convolution for 64 output channels with the workgroup's weights in LDS.
+// Up to 64 input channels (STREAM = false) every weight of the workgroup's 64
+// output channels stays resident for the whole kernel (9 x 64 x Cin bf16,
+// loaded once), so that the per-tile work is the input tile only; above that
+// (STREAM = true) LDS holds one 32-channel chunk of weights, loaded into
+// registers together with the input tile of the next item and stored after the
+// same barrier.  Workgroup: 8 waves, 4 x 2 grid, output tile 16 rows x 32
+// pixels x 64 channels; wave (wr, wc) owns rows 4 wr .. 4 wr + 3 and pixels
 // 16 wc .. 16 wc + 15.  Persistent over items (tile, 32-channel chunk) like
 // conv3_rows_kernel, the next item's 18 x 34-pixel tile loaded into registers
 // during the current item's MFMAs; LDS rows 40 pixels apart, so fragment reads
-// are immediate offsets from per-lane bases.  One workgroup per CU (120 KB of
-// LDS), two waves per SIMD.  Same modes and results (up to the FP32 summation
-// order) as conv3_mfma_kernel.
+// are immediate offsets from per-lane bases.  One workgroup per CU (83-120 KB
+// of LDS), two waves per SIMD.  Same modes and results (up to the FP32
+// summation order) as conv3_mfma_kernel.
 constexpr int kXTileH = 16, kXTileW = 32, kXInRows = kXTileH + 2, kXInW = kXTileW + 2,
               kXPitch = kXTileW + 8;
 constexpr int kXSlots = kXInRows * kXPitch * 4;  // 16-byte slots of the tile
 constexpr int kXLd = (kXSlots + 511) / 512;       // per thread
 constexpr int kXMaxCin = 64;
+constexpr int kXWLd = (9 * 64 * 4 + 511) / 512;   // weight slots of one chunk per thread
 
-template <int MODE>
+template <int MODE, bool STREAM>
 __global__ __launch_bounds__(512, 1) void conv3_res64_kernel(
     const __bf16* __restrict__ X1, int C1, const __bf16* __restrict__ X2, int C2,
     const __bf16* __restrict__ Wt, const float* __restrict__ bias,
     const __bf16* __restrict__ mask, __bf16* __restrict__ Y, int n, int H, int W, int Cout) {
   constexpr int NCO = 64, Q = 4, R = 4;
   __shared__ __attribute__((aligned(16))) char sIn[kXInRows * kXPitch * 64];
-  __shared__ __attribute__((aligned(16))) char sW[(kXMaxCin / 32) * 9 * NCO * 64];
+  __shared__ __attribute__((aligned(16))) char sW[(STREAM ? 1 : kXMaxCin / 32) * 9 * NCO * 64];
   const int Cin = C1 + C2, nch = Cin / 32;
   const int tilesx = (W + kXTileW - 1) / kXTileW;
   const int tilesy = H / kXTileH;
@@ -545,7 +549,7 @@ __global__ __launch_bounds__(512, 1) void conv3_res64_kernel(
   const int sbase = swz_b(tid >> 2, sg);
 
   // all weights of the workgroup's channels: LDS pixel (c 9 + t) NCO + co
-  {
+  if constexpr (!STREAM) {
     const int nslot = nch * 9 * NCO * 4;
     for (int i = tid; i < nslot; i += 512) {
       const int g = i & 3, px = i >> 2;
@@ -564,8 +568,27 @@ __global__ __launch_bounds__(512, 1) void conv3_res64_kernel(
     live |= (sp < kXInRows * kXPitch && rx < kXInW ? 1u : 0u) << r;
     off1[r] = (ry * W + rx) * C1 + sg * 8;
   }
+  // STREAM: weight slot r of this thread is LDS pixel tid/4 + 128 r of the
+  // chunk, i.e. tap (tid >> 8) + 2 r of output channel (tid >> 2) & 63, group
+  // sg, stored at the same sbase + 8192 r as the tile's slots
+  const int wco = (tid >> 2) & 63;
+  const bool wok = co0 + wco < Cout;
+  int woff[kXWLd];
+  unsigned wlive = 0;
+#pragma unroll
+  for (int r = 0; r < kXWLd; ++r) {
+    const int t = (tid >> 8) + 2 * r;
+    wlive |= (t < 9 && wok ? 1u : 0u) << r;
+    woff[r] = ((co0 + wco) * 9 + min(t, 8)) * Cin + sg * 8;
+  }
   bf16x8 ld[kXLd];
+  bf16x8 wl[STREAM ? kXWLd : 1];
   auto load_item = [&](int tile, int ci0) {
+    if constexpr (STREAM) {
+#pragma unroll
+      for (int r = 0; r < kXWLd; ++r)
+        wl[r] = ((wlive >> r) & 1) ? *(const bf16x8*)(Wt + woff[r] + ci0) : zero8();
+    }
     const int img = tile / (tilesx * tilesy);
     const int y0 = ((tile / tilesx) % tilesy) * kXTileH, x0 = (tile % tilesx) * kXTileW;
     const bool interior = y0 >= 1 && y0 + kXTileH + 1 <= H && x0 >= 1 && x0 + kXTileW + 1 <= W;
@@ -604,6 +627,11 @@ __global__ __launch_bounds__(512, 1) void conv3_res64_kernel(
 #pragma unroll
       for (int r = 0; r < kXLd; ++r)
         if ((live >> r) & 1) *(bf16x8*)(sIn + sbase + 8192 * r) = ld[r];
+      if constexpr (STREAM) {
+#pragma unroll
+        for (int r = 0; r < kXWLd; ++r)
+          if ((tid >> 8) + 2 * r < 9) *(bf16x8*)(sW + sbase + 8192 * r) = wl[r];
+      }
       __syncthreads();
       {  // the next item's loads, in flight during this item's MFMAs
         int nt = tile, nc = c + 1;
@@ -613,7 +641,7 @@ __global__ __launch_bounds__(512, 1) void conv3_res64_kernel(
         }
         if (nt < ntiles) load_item(nt, nc * 32);
       }
-      const char* wch = sW + c * 9 * NCO * 64;
+      const char* wch = STREAM ? sW : sW + c * 9 * NCO * 64;
 #pragma unroll
       for (int dx = 0; dx < 3; ++dx) {
         bf16x8 wf[3][Q];
@@ -1062,8 +1090,10 @@ hipError_t unet_conv3(const __bf16* X1, int C1, const __bf16* X2, int C2, const 
                          bias, mask, Y, n, H, W, Cout);
     return hipGetLastError();
   }
-  if (nco == 64 && C1 + C2 <= kXMaxCin && H % kXTileH == 0) {
-    // resident-weight kernel: one 8-wave workgroup per CU
+  if (nco == 64 && H % kXTileH == 0) {
+    // one 8-wave workgroup per CU: weights resident up to 64 input channels;
+    // above that, for the forward pass, streamed per chunk where the tiles fill
+    // every CU
     static int cus = 0;
     if (cus == 0) {
       int dev = 0;
@@ -1072,15 +1102,23 @@ hipError_t unet_conv3(const __bf16* X1, int C1, const __bf16* X2, int C2, const 
         return hipErrorInvalidValue;
     }
     const int xt = n * (H / kXTileH) * ((W + kXTileW - 1) / kXTileW);
-    dim3 gridx(std::min(xt, std::max(1, cus / cob)), cob);
-    if (mode == 0)
-      hipLaunchKernelGGL((conv3_res64_kernel<0>), gridx, dim3(512), 0, s, X1, C1, X2, C2, Wt, bias,
-                         mask, Y, n, H, W, Cout);
-    else
-      hipLaunchKernelGGL((conv3_res64_kernel<1>), gridx, dim3(512), 0, s, X1, C1, X2, C2, Wt, bias,
-                         mask, Y, n, H, W, Cout);
-    return hipGetLastError();
+    const bool resident = C1 + C2 <= kXMaxCin;
+    if (resident || (mode == 0 && xt * cob >= cus)) {
+      dim3 gridx(std::min(xt, std::max(1, cus / cob)), cob);
+#define GLSU_CONV3_RES64(MD, ST)                                                                 \
+  hipLaunchKernelGGL((conv3_res64_kernel<MD, ST>), gridx, dim3(512), 0, s, X1, C1, X2, C2, Wt,   \
+                     bias, mask, Y, n, H, W, Cout)
+      if (resident) {
+        if (mode == 0) GLSU_CONV3_RES64(0, false);
+        else GLSU_CONV3_RES64(1, false);
+      } else {
+        GLSU_CONV3_RES64(0, true);
+      }
+#undef GLSU_CONV3_RES64
+      return hipGetLastError();
+    }
   }
+
   const int tiles = n * (H / kTileH) * ((W + kTileW - 1) / kTileW);
   dim3 grid(tiles, cob);
   if (nco == 64) {

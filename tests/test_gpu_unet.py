@@ -125,6 +125,23 @@ def test_interior_tiles_match_cpu_reference():
             assert rel < 6e-2, k
 
 
+def test_streamed_weight_tiles_match_cpu_reference():
+    """2 x 512 x 776: the level-1 decoder conv (64 + 128 upsampled -> 64
+    channels at 256 x 388) has 416 tiles of 16 x 32 pixels, enough for one
+    workgroup per CU, so the forward pass stages its weights per 32-channel
+    chunk in the persistent kernel (interior, border and a 4-pixel last tile
+    column); forward against the FP64 reference at the bar above."""
+    net, ref = _net_and_ref(seed=10)
+    x, _ = _inputs(2, 512, 776, seed=11)
+    out = net.forward(torch.from_numpy(x).cuda()).cpu().numpy()
+    r = U.forward(ref, _c2(x))
+    corr_ref = r - _c2(x)
+    rel = np.linalg.norm(_c2(out) - r) / np.linalg.norm(corr_ref)
+    rel1 = [np.linalg.norm(_c2(out[i]) - r[i]) / np.linalg.norm(corr_ref[i]) for i in range(2)]
+    say(f"[unet fwd 2x512x776] relative norm {rel:.2e} (per image {rel1[0]:.2e}, {rel1[1]:.2e})")
+    assert rel < 3e-2 and max(rel1) < 3e-2
+
+
 def test_training_reduces_loss():
     x, u = glsu.learned.make_pairs(nseq=1, nframes=3, size=256, seed0=777)
     net = glsu.UNet(4, 32, seed=1)

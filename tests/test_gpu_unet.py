@@ -126,19 +126,22 @@ def test_interior_tiles_match_cpu_reference():
 
 
 def test_streamed_weight_tiles_match_cpu_reference():
-    """2 x 512 x 776: the level-1 decoder conv (64 + 128 upsampled -> 64
-    channels at 256 x 388) has 416 tiles of 16 x 32 pixels, enough for one
-    workgroup per CU, so the forward pass stages its weights per 32-channel
-    chunk in the persistent kernel (interior, border and a 4-pixel last tile
-    column); forward against the FP64 reference at the bar above."""
+    """2 x 512 x 1016: the forward convs of more than 64 input channels whose
+    16 x 32-pixel tiles give every CU a workgroup stage their weights per
+    32-channel chunk in the persistent kernel: the level-1 decoder conv (64 +
+    128 upsampled -> 64 channels at 256 x 508, 512 tiles) and the level-2
+    128 -> 128 and (128 + 256 upsampled) -> 128 convs (128 x 254, 128 tiles x
+    two 64-channel blocks); interior and border tiles, part-width last tile
+    columns (28 and 30 pixels).  Forward against the FP64 reference at the bar
+    above, per image too."""
     net, ref = _net_and_ref(seed=10)
-    x, _ = _inputs(2, 512, 776, seed=11)
+    x, _ = _inputs(2, 512, 1016, seed=11)
     out = net.forward(torch.from_numpy(x).cuda()).cpu().numpy()
     r = U.forward(ref, _c2(x))
     corr_ref = r - _c2(x)
     rel = np.linalg.norm(_c2(out) - r) / np.linalg.norm(corr_ref)
     rel1 = [np.linalg.norm(_c2(out[i]) - r[i]) / np.linalg.norm(corr_ref[i]) for i in range(2)]
-    say(f"[unet fwd 2x512x776] relative norm {rel:.2e} (per image {rel1[0]:.2e}, {rel1[1]:.2e})")
+    say(f"[unet fwd 2x512x1016] relative norm {rel:.2e} (per image {rel1[0]:.2e}, {rel1[1]:.2e})")
     assert rel < 3e-2 and max(rel1) < 3e-2
 
 

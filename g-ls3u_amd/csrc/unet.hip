@@ -138,15 +138,14 @@ static int forward(glsu_unet* h, const float2* x, float2* out, int n, int H, int
   UTRY(unet_conv3_in(x, e0a.w, e0a.b, a.ea[0], n, H, W, h->C(0), s));
   for (int l = 0; l < Lv; ++l) {
     const int Hl = H >> l, Wl = W >> l;
-    if (l > 0) {
-      UTRY(unet_maxpool2(a.e[l - 1], a.p[l - 1], n, H >> (l - 1), W >> (l - 1), h->C(l - 1), s));
+    if (l > 0) {  // a.p[l - 1]: pooled by the previous level's second conv
       Layer& la = h->enc(l, 0);
       UTRY(unet_conv3(a.p[l - 1], h->C(l - 1), nullptr, 0, la.wf, la.b, nullptr, a.ea[l], n, Hl,
                       Wl, h->C(l), 0, s));
     }
     Layer& lb = h->enc(l, 1);
     UTRY(unet_conv3(a.ea[l], h->C(l), nullptr, 0, lb.wf, lb.b, nullptr, a.e[l], n, Hl, Wl,
-                    h->C(l), 0, s));
+                    h->C(l), 0, s, l + 1 < Lv ? a.p[l] : nullptr));
   }
   const __bf16* low = a.e[Lv - 1];
   for (int l = Lv - 2; l >= 0; --l) {

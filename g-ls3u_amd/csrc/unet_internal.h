@@ -7,10 +7,12 @@ namespace glsu {
 
 // 3x3 conv on BF16 MFMA over cat(X1, up2(X2)) (C1, C2 multiples of 32; X2 may
 // be null with C2 = 0).  mode 0: Y = relu(conv + bias) with Wt [Cout][9][Cin];
-// mode 1: Y = conv * (mask > 0) (mask may be null), no bias.
+// mode 1: Y = conv * (mask > 0) (mask may be null), no bias.  pool (mode 0,
+// H and W even): also the 2 x 2 max-pool of Y, n x H/2 x W/2 x Cout, equal to
+// unet_maxpool2(Y) (written from the conv's epilogue where the kernel allows).
 hipError_t unet_conv3(const __bf16* X1, int C1, const __bf16* X2, int C2, const __bf16* Wt,
                       const float* bias, const __bf16* mask, __bf16* Y, int n, int H, int W,
-                      int Cout, int mode, hipStream_t s);
+                      int Cout, int mode, hipStream_t s, __bf16* pool = nullptr);
 // dW [Cout][9][Cin] += weight gradient (FP32, atomically accumulated)
 hipError_t unet_conv3_dw(const __bf16* dY, int Cout, const __bf16* X1, int C1, const __bf16* X2,
                          int C2, float* dW, int n, int H, int W, hipStream_t s);

@@ -314,11 +314,31 @@ struct RowsConv {
   static constexpr int Wl = (WSlots + 255) / 256;
 };
 
+// 2 x 2 max-pool of a ReLU conv output still held as MFMA accumulators: rows
+// o and o + 1 are two registers of the lane (a, b), pixels x and x + 1 two
+// adjacent lanes (lr even / odd).  Every value is rounded to bf16 before the
+// maximum, so the result equals maxpool2_kernel's on the stored output.  All
+// 64 lanes must call it (lane exchange); the even lanes hold the result.
+__device__ __forceinline__ bf16x4 pool_pair(const floatx4& a, const floatx4& b,
+                                            const float (&bv)[4]) {
+  bf16x4 r;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float u = (float)(__bf16)fmaxf(a[j] + bv[j], 0.f);
+    const float v = (float)(__bf16)fmaxf(b[j] + bv[j], 0.f);
+    float m = fmaxf(u, v);
+    m = fmaxf(m, __shfl_xor(m, 1));
+    r[j] = (__bf16)m;
+  }
+  return r;
+}
+
 template <int NCO, int MODE>
 __global__ __launch_bounds__(256, 2) void conv3_rows_kernel(
     const __bf16* __restrict__ X1, int C1, const __bf16* __restrict__ X2, int C2,
     const __bf16* __restrict__ Wt, const float* __restrict__ bias,
-    const __bf16* __restrict__ mask, __bf16* __restrict__ Y, int n, int H, int W, int Cout) {
+    const __bf16* __restrict__ mask, __bf16* __restrict__ Y, int n, int H, int W, int Cout,
+    __bf16* __restrict__ pool) {
   static_assert(NCO == 32, "64-channel tiles use conv3_mfma_kernel");
   using K = RowsConv<NCO>;
   constexpr int Q = K::Q, P = K::PX, R = 4;
@@ -501,6 +521,26 @@ __global__ __launch_bounds__(256, 2) void conv3_rows_kernel(
           *(bf16x4*)(yrow + lo) = out;
         }
     }
+    if constexpr (MODE == 0) {
+      if (pool) {  // fused 2 x 2 max-pool (H, W even; rows y0 + 4 wr + o, o even, pair up)
+        const int H2 = H >> 1, W2 = W >> 1;
+#pragma unroll
+        for (int o = 0; o < R; o += 2) {
+          const int y2 = (y0 + 4 * wr + o) >> 1;
+#pragma unroll
+          for (int p = 0; p < P; ++p) {
+            const int x = x0 + 16 * P * wc + 16 * p + lr;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+              const int cb = co0 + 16 * q + 4 * lk;
+              const bf16x4 m = pool_pair(acc[o][p][q], acc[o + 1][p][q], bv[q]);
+              if (!(lr & 1) && x < W && cb < Cout)
+                *(bf16x4*)(pool + (((size_t)img * H2 + y2) * W2 + (x >> 1)) * Cout + cb) = m;
+            }
+          }
+        }
+      }
+    }
   }
 }
 
@@ -530,7 +570,8 @@ template <int MODE, bool STREAM>
 __global__ __launch_bounds__(512, 1) void conv3_res64_kernel(
     const __bf16* __restrict__ X1, int C1, const __bf16* __restrict__ X2, int C2,
     const __bf16* __restrict__ Wt, const float* __restrict__ bias,
-    const __bf16* __restrict__ mask, __bf16* __restrict__ Y, int n, int H, int W, int Cout) {
+    const __bf16* __restrict__ mask, __bf16* __restrict__ Y, int n, int H, int W, int Cout,
+    __bf16* __restrict__ pool) {
   constexpr int NCO = 64, Q = 4, R = 4;
   __shared__ __attribute__((aligned(16))) char sIn[kXInRows * kXPitch * 64];
   __shared__ __attribute__((aligned(16))) char sW[(STREAM ? 1 : kXMaxCin / 32) * 9 * NCO * 64];
@@ -695,6 +736,22 @@ __global__ __launch_bounds__(512, 1) void conv3_res64_kernel(
           out[j] = (__bf16)v;
         }
         *(bf16x4*)(Y + off) = out;
+      }
+    }
+    if constexpr (MODE == 0) {
+      if (pool) {  // fused 2 x 2 max-pool (H, W even; rows y0 + 4 wr + o, o even, pair up)
+        const int H2 = H >> 1, W2 = W >> 1;
+#pragma unroll
+        for (int o = 0; o < R; o += 2) {
+          const int y2 = (y0 + 4 * wr + o) >> 1;
+#pragma unroll
+          for (int q = 0; q < Q; ++q) {
+            const int cb = co0 + q * 16 + 4 * lk;
+            const bf16x4 m = pool_pair(acc[o][q], acc[o + 1][q], bv[q]);
+            if (!(lr & 1) && x < W && cb < Cout)
+              *(bf16x4*)(pool + (((size_t)img * H2 + y2) * W2 + (x >> 1)) * Cout + cb) = m;
+          }
+        }
       }
     }
   }
@@ -1066,8 +1123,9 @@ static unsigned blocks(long work, int per = 256) { return (unsigned)((work + per
 
 hipError_t unet_conv3(const __bf16* X1, int C1, const __bf16* X2, int C2, const __bf16* Wt,
                       const float* bias, const __bf16* mask, __bf16* Y, int n, int H, int W,
-                      int Cout, int mode, hipStream_t s) {
-  if ((C1 % 32) || (C2 % 32) || (Cout % 16) || (H % kTileH) || (C2 && ((H | W) & 1)))
+                      int Cout, int mode, hipStream_t s, __bf16* pool) {
+  if ((C1 % 32) || (C2 % 32) || (Cout % 16) || (H % kTileH) || (C2 && ((H | W) & 1)) ||
+      (pool && (mode != 0 || ((H | W) & 1))))
     return hipErrorInvalidValue;
   const int nco = Cout >= 64 ? 64 : 32;
   const int cob = (Cout + nco - 1) / nco;
@@ -1084,10 +1142,10 @@ hipError_t unet_conv3(const __bf16* X1, int C1, const __bf16* X2, int C2, const 
     dim3 grid(std::min(rtiles, std::max(1, 2 * cus / cob)), cob);
     if (mode == 0)
       hipLaunchKernelGGL((conv3_rows_kernel<32, 0>), grid, dim3(256), 0, s, X1, C1, X2, C2, Wt,
-                         bias, mask, Y, n, H, W, Cout);
+                         bias, mask, Y, n, H, W, Cout, pool);
     else
       hipLaunchKernelGGL((conv3_rows_kernel<32, 1>), grid, dim3(256), 0, s, X1, C1, X2, C2, Wt,
-                         bias, mask, Y, n, H, W, Cout);
+                         bias, mask, Y, n, H, W, Cout, nullptr);
     return hipGetLastError();
   }
   if (nco == 64 && H % kXTileH == 0) {
@@ -1107,7 +1165,7 @@ hipError_t unet_conv3(const __bf16* X1, int C1, const __bf16* X2, int C2, const 
       dim3 gridx(std::min(xt, std::max(1, cus / cob)), cob);
 #define GLSU_CONV3_RES64(MD, ST)                                                                 \
   hipLaunchKernelGGL((conv3_res64_kernel<MD, ST>), gridx, dim3(512), 0, s, X1, C1, X2, C2, Wt,   \
-                     bias, mask, Y, n, H, W, Cout)
+                     bias, mask, Y, n, H, W, Cout, pool)
       if (resident) {
         if (mode == 0) GLSU_CONV3_RES64(0, false);
         else GLSU_CONV3_RES64(1, false);
@@ -1136,7 +1194,10 @@ hipError_t unet_conv3(const __bf16* X1, int C1, const __bf16* X2, int C2, const 
       hipLaunchKernelGGL((conv3_mfma_kernel<32, 1>), grid, dim3(256), 0, s, X1, C1, X2, C2, Wt,
                          bias, mask, Y, n, H, W, Cout);
   }
-  return hipGetLastError();
+  // this kernel has no pooled epilogue
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !pool) return e;
+  return unet_maxpool2(Y, pool, n, H, W, Cout, s);
 }
 
 hipError_t unet_conv3_dw(const __bf16* dY, int Cout, const __bf16* X1, int C1, const __bf16* X2,

@@ -538,7 +538,7 @@ struct glsu_dpra {
   std::map<GraphKey, GraphEntry> graphs;
   // learned path (glsu_dpra_frame_learned)
   float2* d_net = nullptr;                      // network output, H*W
-  float2 *d_zwin = nullptr, *d_fwin = nullptr;  // spot-check window in / filtered
+  float2* d_fwin = nullptr;  // spot-check window, filtered
   size_t win_cap = 0;
   std::map<std::pair<int, int>, WffPlan> wins;  // WFF2 plans by window size
   int* d_flags = nullptr;                       // [fallback, !fallback] of the last frame
@@ -573,7 +573,7 @@ static void dpra_free(glsu_dpra* h) {
   hipFree(h->phiref); hipFree(h->phicurr);
   hipFree(h->dref_hi); hipFree(h->dref_lo); hipFree(h->dphi_hi); hipFree(h->dphi_lo);
   hipFree(h->z); hipFree(h->d_stage); hipFree(h->d_dphi_stage);
-  hipFree(h->d_net); hipFree(h->d_zwin); hipFree(h->d_fwin); hipFree(h->d_flags);
+  hipFree(h->d_net); hipFree(h->d_fwin); hipFree(h->d_flags);
   hipFree(h->d_stats);
   if (h->h_res) hipHostFree(h->h_res);
   for (auto& kv : h->wins) wff_plan_free(kv.second);
@@ -1380,20 +1380,19 @@ extern "C" int glsu_dpra_frame_learned(glsu_dpra* h, glsu_unet* net, const void*
   WffPlan& wp = it->second;
   if ((size_t)Ww * Hw > h->win_cap) {
     HIP_TRY(hipStreamSynchronize(h->s));
-    hipFree(h->d_zwin);
     hipFree(h->d_fwin);
-    h->d_zwin = h->d_fwin = nullptr;
+    h->d_fwin = nullptr;
     h->win_cap = (size_t)Ww * Hw;
-    HIP_TRY(hipMalloc((void**)&h->d_zwin, h->win_cap * sizeof(float2)));
     HIP_TRY(hipMalloc((void**)&h->d_fwin, h->win_cap * sizeof(float2)));
   }
-  // the window uses the full image's threshold
-  HIP_TRY(hipMemcpyAsync(wp.d_thr, p.d_thr, sizeof(float), hipMemcpyDeviceToDevice, h->s));
-  wp.thr_ready = true;
-  HIP_TRY(hipMemcpy2DAsync(h->d_zwin, Ww * sizeof(float2), h->z + (size_t)y0 * W + x0,
-                           W * sizeof(float2), Ww * sizeof(float2), Hw, hipMemcpyDeviceToDevice,
-                           h->s));
-  HIP_TRY(launch_rows_fwd(wp, h->d_zwin, h->s));
+  // the window uses the full image's threshold, fixed once computed (it is
+  // never recomputed on this handle): copied once per window plan
+  if (!wp.thr_ready) {
+    HIP_TRY(hipMemcpyAsync(wp.d_thr, p.d_thr, sizeof(float), hipMemcpyDeviceToDevice, h->s));
+    wp.thr_ready = true;
+  }
+  // the window's rows are read in place from the full field (row pitch W)
+  HIP_TRY(launch_rows_fwd(wp, h->z + (size_t)y0 * W + x0, h->s, nullptr, W));
   HIP_TRY(launch_cols(wp, h->s));
   HIP_TRY(launch_rows_inv_filtered(wp, h->d_fwin, h->s));
   HIP_TRY(launch_tile_residual(h->d_net, W, tx, ty, h->d_fwin, Ww, tx - x0, ty - y0, (float)tol,

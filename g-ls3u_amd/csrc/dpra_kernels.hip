@@ -157,28 +157,37 @@ hipError_t launch_net_update(const float2* out, const UpdateArgs& u, size_t n, h
 
 // RMS over a 64 x 64 tile of wrap(angle(net) - angle(fwin)); sets the
 // fallback flag (and its complement) and accumulates the statistics
-__global__ __launch_bounds__(256) void tile_residual_kernel(
+__global__ __launch_bounds__(1024) void tile_residual_kernel(
     const float2* __restrict__ net, int W, int tx, int ty, const float2* __restrict__ fwin, int Ww,
     int ox, int oy, float tol, int* __restrict__ flag, int* __restrict__ not_flag,
     float* __restrict__ stats) {
-  __shared__ float red[256];
+  // 4096 pixels, four per thread: all eight loads are issued before the first
+  // atan2, then the squares are summed per wave (lane exchange) and across
+  // the 16 waves in LDS
+  __shared__ float red[16];
+  float2 a[4], b[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int i = threadIdx.x + 1024 * k, r = i / 64, c = i % 64;
+    a[k] = net[(size_t)(ty + r) * W + tx + c];
+    b[k] = fwin[(size_t)(oy + r) * Ww + ox + c];
+  }
   float s = 0.f;
-  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
-    const int r = i / 64, c = i % 64;
-    const float2 a = net[(size_t)(ty + r) * W + tx + c];
-    const float2 b = fwin[(size_t)(oy + r) * Ww + ox + c];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
     // angle(a conj(b)) = wrapped phase difference
-    const float d = atan2f(a.y * b.x - a.x * b.y, a.x * b.x + a.y * b.y);
+    const float d = atan2f(a[k].y * b[k].x - a[k].x * b[k].y, a[k].x * b[k].x + a[k].y * b[k].y);
     s += d * d;
   }
-  red[threadIdx.x] = s;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = s;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
-    const float rms = sqrtf(red[0] / 4096.f);
+    float t = 0.f;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) t += red[w];
+    const float rms = sqrtf(t / 4096.f);
     const int fb = rms > tol ? 1 : 0;
     *flag = fb;
     *not_flag = 1 - fb;
@@ -193,7 +202,7 @@ __global__ __launch_bounds__(256) void tile_residual_kernel(
 hipError_t launch_tile_residual(const float2* net, int W, int tx, int ty, const float2* fwin,
                                 int Ww, int ox, int oy, float tol, int* flag, int* not_flag,
                                 float* stats, hipStream_t s) {
-  hipLaunchKernelGGL(tile_residual_kernel, dim3(1), dim3(256), 0, s, net, W, tx, ty, fwin, Ww, ox,
+  hipLaunchKernelGGL(tile_residual_kernel, dim3(1), dim3(1024), 0, s, net, W, tx, ty, fwin, Ww, ox,
                      oy, tol, flag, not_flag, stats);
   return hipGetLastError();
 }

@@ -99,9 +99,10 @@ hipError_t launch_plan_tables(const WffPlan& p, double* gx, double* gy, hipStrea
 
 // Kernel launchers (all asynchronous on `s`).
 hipError_t launch_thr(const WffPlan& p, const float2* f, hipStream_t s);
-// skip (optional device flag): when *skip != 0 the kernel exits at once
+// skip (optional device flag): when *skip != 0 the kernel exits at once;
+// ld: elements between rows of f (0: p.W), e.g. a window of a wider field
 hipError_t launch_rows_fwd(const WffPlan& p, const float2* f, hipStream_t s,
-                           const int* skip = nullptr);
+                           const int* skip = nullptr, int ld = 0);
 hipError_t launch_cols(const WffPlan& p, hipStream_t s, const int* skip = nullptr);
 // mode 0: out = filtered complex field (H*W float2)
 hipError_t launch_rows_inv_filtered(const WffPlan& p, float2* out, hipStream_t s);

@@ -146,14 +146,14 @@ __device__ __forceinline__ void rows_fwd_body(const float2* __restrict__ frow,
 
 template <class P, int EV = P::E>
 __global__ __launch_bounds__(64 * kRowWavesMax) void wff_rows_fwd_kernel(
-    const float2* __restrict__ f, float2* __restrict__ A, const float* __restrict__ Hx,
+    const float2* __restrict__ f, int ld, float2* __restrict__ A, const float* __restrict__ Hx,
     const float2* __restrict__ tw_g, int W, int H, int nwx, size_t plane,
     const int* __restrict__ skip) {
   extern __shared__ float2 smem[];
   if (skip && *skip) return;  // device-side decision (learned path): not needed
   const int y = xcd_remap(blockIdx.x, gridDim.x);  // uniform per workgroup
   if (y >= H) return;
-  rows_fwd_body<P, EV>(f + (size_t)y * W, A, Hx, tw_g, W, H, nwx, plane, y, threadIdx.x % P::T,
+  rows_fwd_body<P, EV>(f + (size_t)y * ld, A, Hx, tw_g, W, H, nwx, plane, y, threadIdx.x % P::T,
                    smem);
 }
 

@@ -3,7 +3,10 @@
 
 namespace glsu {
 
-hipError_t launch_rows_fwd(const WffPlan& p, const float2* f, hipStream_t s, const int* skip) {
+hipError_t launch_rows_fwd(const WffPlan& p, const float2* f, hipStream_t s, const int* skip,
+                           int ld) {
+  if (ld == 0) ld = p.W;
+  if (ld < p.W) return hipErrorInvalidValue;
   const int rw = p.rows_waves;
   const int smem = plan_smem_bytes(p.planx, rw);
   switch (p.planx) {
@@ -11,10 +14,10 @@ hipError_t launch_rows_fwd(const WffPlan& p, const float2* f, hipStream_t s, con
   if constexpr (P::T == 64) {                                                               \
     if (live_regs_fit<P>(p.W))                                                              \
       hipLaunchKernelGGL((wff_rows_fwd_kernel<P, kLiveRegs<P>>), dim3(p.H), dim3(64 * rw),  \
-                         smem, s, f, p.d_A, p.d_Hx, p.d_twx, p.W, p.H, p.nwx,               \
+                         smem, s, f, ld, p.d_A, p.d_Hx, p.d_twx, p.W, p.H, p.nwx,           \
                          p.plane_elems, skip);                                              \
     else                                                                                    \
-      hipLaunchKernelGGL((wff_rows_fwd_kernel<P>), dim3(p.H), dim3(64 * rw), smem, s, f,    \
+      hipLaunchKernelGGL((wff_rows_fwd_kernel<P>), dim3(p.H), dim3(64 * rw), smem, s, f, ld, \
                          p.d_A, p.d_Hx, p.d_twx, p.W, p.H, p.nwx, p.plane_elems, skip);     \
   } else {                                                                                  \
     return hipErrorInvalidValue;                                                            \

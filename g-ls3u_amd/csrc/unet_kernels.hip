@@ -339,7 +339,7 @@ __global__ __launch_bounds__(256, 2) void conv3_rows_kernel(
     const __bf16* __restrict__ Wt, const float* __restrict__ bias,
     const __bf16* __restrict__ mask, __bf16* __restrict__ Y, int n, int H, int W, int Cout,
     __bf16* __restrict__ pool) {
-  static_assert(NCO == 32, "64-channel tiles use conv3_mfma_kernel");
+  static_assert(NCO == 32, "64-channel tiles use conv3_res64_kernel or conv3_mfma_kernel");
   using K = RowsConv<NCO>;
   constexpr int Q = K::Q, P = K::PX, R = 4;
   constexpr int kRTileW = K::TW, kRInW = K::InW, kRPitch = K::Pitch;

@@ -878,8 +878,9 @@ __global__ __launch_bounds__(1024) void conv3_in_dw_kernel(const float2* __restr
                                                           float* __restrict__ dW,
                                                           float* __restrict__ db, int n, int H,
                                                           int W, int C0, int pix_per_block) {
-  // thread k < C0*19 owns one weight (or bias) and sums it over the block's pixels
-  const int k = threadIdx.x;
+  // thread k < C0*19 (k over blockIdx.y, threadIdx.x) owns one weight (or
+  // bias) and sums it over the block's pixels
+  const int k = blockIdx.y * blockDim.x + threadIdx.x;
   const long P = (long)n * H * W;
   const long p0 = (long)blockIdx.x * pix_per_block, p1 = std::min(P, p0 + pix_per_block);
   if (k >= C0 * 19) return;
@@ -1222,12 +1223,12 @@ hipError_t unet_conv3_in(const float2* x, const float* w, const float* b, __bf16
 
 hipError_t unet_conv3_in_dw(const float2* x, const __bf16* dY, float* dW, float* db, int n,
                             int H, int W, int C0, hipStream_t s) {
-  if (C0 * 19 > 1024) return hipErrorInvalidValue;
   const long P = (long)n * H * W;
   const int ppb = 2048;
-  const int threads = ((C0 * 19 + 63) / 64) * 64;
-  hipLaunchKernelGGL(conv3_in_dw_kernel, dim3(blocks(P, ppb)), dim3(threads), 0, s, x, dY, dW,
-                     db, n, H, W, C0, ppb);
+  const int nk = C0 * 19;  // weights and biases
+  const int threads = std::min(1024, ((nk + 63) / 64) * 64);
+  hipLaunchKernelGGL(conv3_in_dw_kernel, dim3(blocks(P, ppb), (nk + threads - 1) / threads),
+                     dim3(threads), 0, s, x, dY, dW, db, n, H, W, C0, ppb);
   return hipGetLastError();
 }
 

@@ -145,6 +145,38 @@ def test_streamed_weight_tiles_match_cpu_reference():
     assert rel < 3e-2 and max(rel1) < 3e-2
 
 
+@pytest.mark.parametrize("levels,base,shape", [(3, 64, (1, 64, 96)), (2, 32, (2, 48, 80))])
+def test_other_depths_and_widths_match_cpu_reference(levels, base, shape):
+    """Networks other than 4 levels x 32 channels: with 64 base channels the
+    full-resolution convs run on the 64-channel kernels (the first encoder's
+    max-pool from a conv epilogue) and the output layer sums 64 channels;
+    with 2 levels a single pooling step.  Forward and every gradient against
+    the FP64 reference at the bars above."""
+    net, ref = _net_and_ref(seed=12, levels=levels, base=base)
+    n, H, W = shape
+    x, u = _inputs(n, H, W, seed=13)
+    out = net.forward(torch.from_numpy(x).cuda()).cpu().numpy()
+    r = U.forward(ref, _c2(x), levels=levels)
+    corr_ref = r - _c2(x)
+    rel = np.linalg.norm(_c2(out) - r) / np.linalg.norm(corr_ref)
+    say(f"[unet fwd L{levels} b{base} {shape}] relative norm {rel:.2e}")
+    assert rel < 3e-2
+    loss = net.loss_grad(torch.from_numpy(x).cuda(), torch.from_numpy(u).cuda())
+    g = net.grads()
+    Lr, gr = U.loss_and_grads(ref, _c2(x), _c2(u), levels=levels)
+    say(f"[unet grad L{levels} b{base}] loss gpu {loss:.6f} cpu {Lr:.6f}")
+    assert abs(loss - Lr) / Lr < 1e-2
+    for k in gr:
+        a, b = g[k].astype(np.float64).ravel(), gr[k].ravel()
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        cos = float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30))
+        say(f"   {k:8s} rel err {rel:.2e} cos {cos:.5f}")
+        if k.startswith("e0a"):
+            assert rel < 0.2 and cos > 0.98, k
+        else:
+            assert rel < 6e-2, k
+
+
 def test_training_reduces_loss():
     x, u = glsu.learned.make_pairs(nseq=1, nframes=3, size=256, seed0=777)
     net = glsu.UNet(4, 32, seed=1)

@@ -146,13 +146,14 @@ def test_c5_learned_chain_with_committed_weights(oracle):
         assert raw_rms / chain_rms > 5.0
 
 
-def test_spot_check_residual_equals_full_frame_wff2():
+@pytest.mark.parametrize("W,H", [(512, 512), (640, 384)])
+def test_spot_check_residual_equals_full_frame_wff2(W, H):
     """The fallback's residual is computed from WFF2 on the tile's window
-    (tile +- 2s, clipped); it must equal the RMS of angle(net * conj(F)) on
-    the tile with F the full-frame WFF2 of the same raw field."""
+    (tile +- 2s, clipped; its rows read in place from the W-wide field); it
+    must equal the RMS of angle(net * conj(F)) on the tile with F the
+    full-frame WFF2 of the same raw field, on square and non-square images."""
     path = glsu.learned.WEIGHTS
     net = glsu.UNet.load(path) if os.path.exists(path) else glsu.UNet(4, 32, seed=4)
-    W = H = 512
     params = glsu.wff_params(10.0, -0.2, 0.2, 0.1)
     frames = glsu.gen_frames(W, H, 1, 4711, amp0=3.5, noise=15.0)
     phi0 = glsu.synth.phase(W, H, 3.5, 0.05)

@@ -78,9 +78,7 @@ class WFT2:
         self._check_f(f)
         if out is None:
             out = torch.empty((8, self.height, self.width), dtype=torch.float32, device=f.device)
-        if out.dtype != torch.float32 or tuple(out.shape) != (8, self.height, self.width) or \
-                not out.is_contiguous():
-            raise ValueError("out must be a contiguous (8, height, width) float32 tensor")
+        check_out(out, (8, self.height, self.width), torch.float32)
         z = WfrResults(*[out[i].data_ptr() for i in range(8)])
         ms = ctypes.c_double()
         check(lib().glsu_wft2_compute_wfr(self._h, ctypes.c_void_p(f.data_ptr()), ctypes.byref(z),
@@ -99,10 +97,7 @@ class WFT2:
         filtered field; WFR mode: the dict of wfr()."""
         if self.type == WFR:
             return self.wfr(f, out, timed)
-        if f.dtype != torch.complex64 or not f.is_cuda or not f.is_contiguous():
-            raise ValueError("f must be a contiguous complex64 CUDA tensor")
-        if tuple(f.shape) != (self.height, self.width):
-            raise ValueError(f"f has shape {tuple(f.shape)}, expected {(self.height, self.width)}")
+        self._check_f(f)
         if out is None:
             out = torch.empty_like(f)
         check_out(out, (self.height, self.width), torch.complex64)

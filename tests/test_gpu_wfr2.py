@@ -118,3 +118,26 @@ def test_wfr2_host_entry_determinism_and_errors():
     wf = glsu.WFT2(W, H, params=p)
     with pytest.raises(glsu.GlsuError):
         wf.wfr(torch.from_numpy(f).cuda())
+
+
+def test_wft2_outputs_checked_before_launch():
+    """Outputs of the wrong device, shape or dtype are refused in Python before
+    their address reaches the library, in both modes."""
+    W, H = 120, 90
+    p = glsu.wff_params(5.0, -0.4, 0.4, 0.1)
+    f = torch.from_numpy(field(W, H, 7)).cuda()
+    w = glsu.WFT2(W, H, type=glsu.WFR, params=p)
+    say("[wft2 out checks] 120x90 WFR and WFF")
+    for bad in (torch.empty((8, H, W), dtype=torch.float32),
+                torch.empty((8, H, W + 1), dtype=torch.float32, device="cuda"),
+                torch.empty((8, H, W), dtype=torch.float64, device="cuda")):
+        with pytest.raises(ValueError):
+            w.wfr(f, out=bad)
+    wf = glsu.WFT2(W, H, params=p)
+    for bad in (torch.empty((H, W), dtype=torch.complex64),
+                torch.empty((H, W), dtype=torch.float32, device="cuda"),
+                torch.empty((W, H), dtype=torch.complex64, device="cuda").t()):
+        with pytest.raises(ValueError):
+            wf(f, out=bad)
+    good = wf(f)
+    assert good.shape == (H, W) and bool(torch.isfinite(torch.view_as_real(good)).all())

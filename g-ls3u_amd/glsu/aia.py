@@ -14,7 +14,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import F32, F64, U8, check, lib
+from ._lib import F32, F64, U8, check, check_out, lib
 
 ERR_ABS, ERR_REL = 0, 1
 _DT = {torch.uint8: U8, torch.float32: F32, torch.float64: F64}
@@ -59,8 +59,9 @@ class AIA:
         """Average ms over `reps` repetitions with the convergence test off of
         one iteration (stage 0: pass + solve launches) or of the pass kernel
         alone (stage 1)."""
-        if frames.dtype not in _DT or tuple(frames.shape) != (self.M, self.H, self.W):
-            raise ValueError("frames must be a (M, H, W) uint8/float32/float64 CUDA tensor")
+        if (frames.dtype not in _DT or tuple(frames.shape) != (self.M, self.H, self.W)
+                or not frames.is_cuda or not frames.is_contiguous()):
+            raise ValueError("frames must be a contiguous (M, H, W) uint8/float32/float64 CUDA tensor")
         d = self._deltas(deltas)
         ms = ctypes.c_double()
         check(lib().glsu_aia_time_iter(self._h, ctypes.c_void_p(frames.data_ptr()),
@@ -90,6 +91,7 @@ class AIA:
             raise ValueError(f"frames must be a contiguous ({self.M}, {self.H}, {self.W}) tensor")
         if phi is None:
             phi = torch.empty((self.H, self.W), dtype=torch.float32, device=frames.device)
+        check_out(phi, (self.H, self.W), torch.float32, "phi")
         d = self._deltas(deltas)
         it, err, ms = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         check(lib().glsu_aia_run(self._h, ctypes.c_void_p(frames.data_ptr()), _DT[frames.dtype],

@@ -207,6 +207,7 @@ def gen_frames(width, height, nframes, seed, amp0=3.0, amp1=0.1, carrier=0.05, n
     the oracle's generator: SplitMix64 + FP64 Box-Muller)."""
     if out is None:
         out = torch.empty((nframes, height, width), dtype=torch.float32, device="cuda")
+    check_out(out, (nframes, height, width), torch.float32)
     s = stream if stream is not None else torch.cuda.current_stream()
     check(lib().glsu_gen_frames(ctypes.c_void_p(out.data_ptr()), int(width), int(height), int(t0),
                                 int(nframes), int(seed), float(amp0), float(amp1), float(carrier),

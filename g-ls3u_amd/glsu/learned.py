@@ -125,6 +125,8 @@ class UNet:
     def loss_grad(self, x, u):
         self._c64(x, "x")
         self._c64(u, "u")
+        if u.shape != x.shape:
+            raise ValueError(f"u has shape {tuple(u.shape)}, expected x's {tuple(x.shape)}")
         n, H, W = x.shape
         loss = ctypes.c_double()
         check(lib().glsu_unet_loss_grad(self._h, ctypes.c_void_p(x.data_ptr()),

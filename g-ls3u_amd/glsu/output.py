@@ -4,7 +4,7 @@ import ctypes
 
 import torch
 
-from ._lib import check, lib
+from ._lib import check, check_out, lib
 
 
 def phase_to_u8(phi, out=None, stream=None):
@@ -14,8 +14,7 @@ def phase_to_u8(phi, out=None, stream=None):
         raise ValueError("phi must be a contiguous float32 CUDA tensor")
     if out is None:
         out = torch.empty(phi.shape, dtype=torch.uint8, device=phi.device)
-    if out.dtype != torch.uint8 or out.shape != phi.shape or not out.is_contiguous():
-        raise ValueError("out must be a contiguous uint8 tensor of phi's shape")
+    check_out(out, phi.shape, torch.uint8)
     s = stream if stream is not None else torch.cuda.current_stream()
     check(lib().glsu_phase_to_u8(ctypes.c_void_p(phi.data_ptr()), ctypes.c_void_p(out.data_ptr()),
                                  phi.numel(), ctypes.c_void_p(s.cuda_stream)))

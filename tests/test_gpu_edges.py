@@ -157,3 +157,31 @@ def test_profile_counts_skipped_column_transforms(oracle):
             assert skipped == expect
         else:
             assert 0 <= skipped < total
+
+
+def test_caller_tensors_checked_before_launch():
+    """Tensors whose raw addresses reach the library (AIA's phase output, the
+    U-Net loss target, the 8-bit output stage, the frame generator's output)
+    are refused in Python when their device, shape or dtype is wrong."""
+    say("[tensor checks] AIA 4x64x48, UNet 2x32, phase_to_u8, gen_frames")
+    M, W, H = 4, 64, 48
+    a = glsu.AIA(M, W, H)
+    frames = torch.zeros((M, H, W), dtype=torch.float32, device="cuda")
+    for bad in (torch.empty((H, W), dtype=torch.float32),
+                torch.empty((H, W + 1), dtype=torch.float32, device="cuda"),
+                torch.empty((H, W), dtype=torch.float64, device="cuda")):
+        with pytest.raises(ValueError):
+            a(frames, phi=bad)
+    with pytest.raises(ValueError):
+        a.time_iter(frames[:, :, :W // 2].contiguous(), None)
+    net = glsu.UNet(2, 32)
+    x = torch.zeros((1, 32, 32), dtype=torch.complex64, device="cuda")
+    with pytest.raises(ValueError):
+        net.loss_grad(x, torch.zeros((1, 32, 16), dtype=torch.complex64, device="cuda"))
+    phi = torch.zeros((H, W), dtype=torch.float32, device="cuda")
+    with pytest.raises(ValueError):
+        glsu.phase_to_u8(phi, out=torch.empty((H, W), dtype=torch.uint8))
+    with pytest.raises(ValueError):
+        glsu.gen_frames(W, H, 2, 1, out=torch.empty((1, H, W), dtype=torch.float32, device="cuda"))
+    assert glsu.phase_to_u8(phi).shape == (H, W)
+    assert glsu.gen_frames(W, H, 2, 1).shape == (2, H, W)
